@@ -1,0 +1,197 @@
+"""ctypes binding of the engine's C ABI (include/hammingdist_amd.h).
+
+This is the same boundary a C, Go or Rust caller binds; tests and the
+benchmark use it to drive the engine directly (device pointers come from
+torch tensors there).  Every call raises ``EngineError`` with the engine's
+message on a non-zero status.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libhd_engine.so")
+
+HD_OK = 0
+HD_ERR_INVALID = 1
+HD_ERR_NO_DEVICE = 2
+HD_ERR_HIP = 3
+HD_ERR_CALLBACK = 4
+HD_ERR_CAPACITY = 5
+
+HD_SEM_DENSE = 0
+HD_SEM_SPARSE = 1
+HD_SEM_SPARSE_X = 2
+HD_SEM_CHAR_X = 3
+
+u8p = C.POINTER(C.c_uint8)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+i32p = C.POINTER(C.c_int32)
+intp = C.POINTER(C.c_int)
+vp = C.c_void_p
+
+BAND_CB = C.CFUNCTYPE(C.c_int, vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64)
+
+# name -> (restype, argtypes); the declarations of include/hammingdist_amd.h
+SIGNATURES = {
+    "hd_last_error": (C.c_char_p, []),
+    "hd_device_count": (C.c_int, []),
+    "hd_device_arch": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "hd_distance_pair": (C.c_int, [vp, vp, C.c_uint64, C.c_int, C.c_int32, i32p]),
+    "hd_all_pairs": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_uint32, vp]),
+    "hd_all_pairs_stream": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
+                                      C.c_uint32, C.c_uint64, BAND_CB, vp]),
+    "hd_reference_distances": (C.c_int, [vp, vp, C.c_uint64, C.c_uint64, vp, C.c_uint64,
+                                         C.c_int, vp]),
+    "hd_consensus_diff": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64, u64p, u32p]),
+    "hd_sparse_emit": (C.c_int, [vp, C.c_uint64, C.c_int, C.c_int64, vp, C.c_uint64, u64p]),
+    "hd_planes_layout": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, u64p,
+                                   u32p, u32p, u64p]),
+    "hd_nibble_mask_dev": (C.c_int, [vp, C.c_uint64, C.c_uint64, u32p, vp]),
+    "hd_encode_dev": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, vp, vp]),
+    "hd_all_pairs_dev": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, C.c_int,
+                                   C.c_uint32, C.c_int, C.c_int, vp, vp]),
+    "hd_reference_distances_dev": (C.c_int, [vp, vp, C.c_uint64, C.c_uint64, vp, C.c_uint64,
+                                             C.c_int, vp, vp]),
+    "hd_variant_count": (C.c_int, []),
+    "hd_variant_info": (C.c_int, [C.c_int, intp, intp, intp, intp]),
+}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            raise ImportError(f"{LIBRARY} is not built; run hammingdist_amd/build.py")
+        handle = C.CDLL(LIBRARY)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != HD_OK:
+        raise EngineError(status, lib().hd_last_error().decode())
+
+
+def ptr(a) -> int:
+    """Address of a numpy array or a torch tensor's storage."""
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def device_count() -> int:
+    return lib().hd_device_count()
+
+
+def device_arch() -> str:
+    buf = C.create_string_buffer(64)
+    check(lib().hd_device_arch(buf, 64))
+    return buf.value.decode()
+
+
+def all_pairs(packed, n: int, bps: int, semantics: int, out_bytes: int, max_dist: int, out):
+    check(lib().hd_all_pairs(ptr(packed), n, bps, semantics, out_bytes, max_dist, ptr(out)))
+
+
+def all_pairs_stream(packed, n, bps, semantics, out_bytes, max_dist, band_cells, callback):
+    """callback(cells_address, first_cell, n_cells, row_begin, row_end) -> None."""
+    errors = []
+
+    def cb(_user, cells, first, count, r0, r1):
+        try:
+            callback(cells, first, count, r0, r1)
+            return 0
+        except Exception as exc:  # reported after the call
+            errors.append(exc)
+            return 1
+
+    fn = BAND_CB(cb)
+    st = lib().hd_all_pairs_stream(ptr(packed), n, bps, semantics, out_bytes, max_dist,
+                                   band_cells, fn, None)
+    if errors:
+        raise errors[0]
+    check(st)
+
+
+def distance_pair(a, b, nbytes: int, semantics: int, max_dist: int) -> int:
+    out = C.c_int32(0)
+    check(lib().hd_distance_pair(ptr(a), ptr(b), nbytes, semantics, max_dist, C.byref(out)))
+    return out.value
+
+
+def reference_distances(query, db, n_db: int, bps: int, rows, m: int, semantics: int, out):
+    check(lib().hd_reference_distances(ptr(query), ptr(db), n_db, bps,
+                                       None if rows is None else ptr(rows), m, semantics,
+                                       ptr(out)))
+
+
+def consensus_diff(packed, n: int, bps: int, L: int) -> tuple[int, int]:
+    nd, mask = C.c_uint64(0), C.c_uint32(0)
+    check(lib().hd_consensus_diff(ptr(packed), n, bps, L, C.byref(nd), C.byref(mask)))
+    return nd.value, mask.value
+
+
+def sparse_emit(cells, n: int, out_bytes: int, threshold: int, triplets, capacity: int) -> int:
+    count = C.c_uint64(0)
+    st = lib().hd_sparse_emit(ptr(cells), n, out_bytes, threshold,
+                              None if triplets is None else ptr(triplets), capacity,
+                              C.byref(count))
+    if st not in (HD_OK, HD_ERR_CAPACITY) or (st == HD_ERR_CAPACITY and capacity):
+        check(st)
+    return count.value
+
+
+def planes_layout(n: int, bps: int, semantics: int, mask: int):
+    n_pad, W, P, total = C.c_uint64(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+    check(lib().hd_planes_layout(n, bps, semantics, mask, C.byref(n_pad), C.byref(W),
+                                 C.byref(P), C.byref(total)))
+    return n_pad.value, W.value, P.value, total.value
+
+
+def nibble_mask_dev(d_packed, n: int, bps: int, stream: int = 0) -> int:
+    mask = C.c_uint32()
+    check(lib().hd_nibble_mask_dev(ptr(d_packed), n, bps, C.byref(mask), stream))
+    return mask.value
+
+
+def encode_dev(d_packed, n: int, bps: int, semantics: int, mask: int, d_planes, stream: int = 0):
+    check(lib().hd_encode_dev(ptr(d_packed), n, bps, semantics, mask, ptr(d_planes), stream))
+
+
+def all_pairs_dev(d_planes, n, bps, semantics, mask, out_bytes, max_dist, early_exit, variant,
+                  d_out, stream: int = 0):
+    check(lib().hd_all_pairs_dev(ptr(d_planes), n, bps, semantics, mask, out_bytes, max_dist,
+                                 1 if early_exit else 0, variant, ptr(d_out), stream))
+
+
+def reference_distances_dev(d_query, d_db, n_db, bps, d_rows, m, semantics, d_out,
+                            stream: int = 0):
+    check(lib().hd_reference_distances_dev(ptr(d_query), ptr(d_db), n_db, bps,
+                                           None if d_rows is None else ptr(d_rows), m,
+                                           semantics, ptr(d_out), stream))
+
+
+def variants() -> list[dict]:
+    out = []
+    for v in range(lib().hd_variant_count()):
+        r, c, w, p = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().hd_variant_info(v, C.byref(r), C.byref(c), C.byref(w), C.byref(p)))
+        out.append({"variant": v, "rows_per_lane": r.value, "cols_per_wave": c.value,
+                    "waves": w.value})
+    return out

@@ -1,0 +1,928 @@
+// HIP engine of the all-pairs Hamming distance path: kernels other than the
+// all-pairs tile kernel (kernels.h), their launchers, and the C ABI declared
+// in include/hammingdist_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "engine_common.h"
+#include "kernels.h"
+
+namespace hd {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+void ensure_device() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    throw Error(HD_ERR_NO_DEVICE, "no HIP device available");
+}
+
+// ---------------------------------------------------------------------------
+// plane codes
+// ---------------------------------------------------------------------------
+// Nibble codes of the packed input (src/hamming_impl.cc:31-44): A=1 C=2 G=4
+// T=8 '-'=15, X=3 when packed with the include_x table, every other char 0.
+static constexpr uint32_t kLetters = (1u << 1) | (1u << 2) | (1u << 4) | (1u << 8);
+static constexpr uint32_t kGapBit = 1u << 15;
+static constexpr uint32_t kXBit = 1u << 3;
+
+PlaneCode plane_code(int semantics, uint32_t mask) {
+  PlaneCode pc;
+  const uint8_t one_hot[4] = {1, 2, 4, 8};
+  switch (semantics) {
+  case HD_SEM_DENSE:
+    // The nibble itself: f = ((a & b) == 0) for every pair of codes.
+    pc.planes = 4;
+    for (int v = 0; v < 16; ++v) pc.table[v] = (uint8_t)v;
+    return pc;
+  case HD_SEM_SPARSE:
+  case HD_SEM_SPARSE_X: {
+    // f = codes differ and neither is '-': one-hot classes, '-' all ones.
+    // X is a class of its own only for SPARSE_X; every code that is not a
+    // letter, '-' or (SPARSE_X) X is the invalid class.  Classes absent from
+    // the data need no plane.
+    const bool x_class = semantics == HD_SEM_SPARSE_X;
+    const uint32_t known = kLetters | kGapBit | (x_class ? kXBit : 0u);
+    const bool has_x = x_class && (mask & kXBit);
+    const bool has_inv = (mask & ~known & 0xFFFFu) != 0;
+    pc.planes = 4 + (has_x ? 1 : 0) + (has_inv ? 1 : 0);
+    const uint8_t all = (uint8_t)((1u << pc.planes) - 1);
+    const uint8_t x_plane = has_x ? (uint8_t)(1u << 4) : 0;
+    const uint8_t inv_plane = has_inv ? (uint8_t)(1u << (4 + (has_x ? 1 : 0))) : 0;
+    for (int v = 0; v < 16; ++v) pc.table[v] = inv_plane;
+    for (int l = 0; l < 4; ++l) pc.table[one_hot[l]] = one_hot[l];
+    if (x_class) pc.table[3] = x_plane;
+    pc.table[15] = all;
+    return pc;
+  }
+  case HD_SEM_CHAR_X: {
+    // f = (a & b) == 0 || (a != b && no '-'): one-hot {A,C,G,T,X}, '-' all
+    // ones, every invalid code all zeros (distance 1 against anything).
+    const bool has_x = (mask & kXBit) != 0;
+    pc.planes = 4 + (has_x ? 1 : 0);
+    for (int v = 0; v < 16; ++v) pc.table[v] = 0;
+    for (int l = 0; l < 4; ++l) pc.table[one_hot[l]] = one_hot[l];
+    pc.table[3] = has_x ? (uint8_t)(1u << 4) : 0;
+    pc.table[15] = (uint8_t)((1u << pc.planes) - 1);
+    return pc;
+  }
+  default:
+    throw Error(HD_ERR_INVALID, "unknown semantics " + std::to_string(semantics));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tile variants of the all-pairs kernel
+// ---------------------------------------------------------------------------
+struct Variant {
+  int R, C, NW;
+};
+// Rows per lane, columns per wave, waves per workgroup.
+static const Variant kVariants[] = {
+    {4, 16, 4}, // 256 x 64 tiles: the large-n shape for 4 planes
+    {8, 8, 4},  // 512 x 32
+    {4, 8, 4},  // 256 x 32: 5-6 planes (fewer SGPRs per column block)
+    {2, 16, 4}, // 128 x 64
+    {2, 8, 4},  // 128 x 32: small datasets
+};
+static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+static int default_variant(int planes, uint64_t n) {
+  if (n < 512) return 4;
+  if (planes == 4) return n < 2048 ? 3 : 0;
+  return 2;
+}
+
+template <int P, int R, int C, int NW, typename OutT>
+static void launch_allpairs_t(const uint32_t *planes, uint32_t n,
+                              uint32_t n_pad, uint32_t W, uint64_t rb0,
+                              uint64_t rb1, OutT *out, uint64_t cell_offset,
+                              uint32_t max_dist, uint32_t check_every,
+                              hipStream_t s) {
+  constexpr uint64_t M = (64 * R) / (NW * C);
+  const uint64_t t0 = tiles_before(rb0, M), t1 = tiles_before(rb1, M);
+  const uint64_t nt = t1 - t0;
+  if (nt == 0) return;
+  if (nt > 0x7fffffffull) throw Error(HD_ERR_INVALID, "too many tiles in one launch");
+  (void)n_pad;
+  hipLaunchKernelGGL((k_allpairs<P, R, C, NW, OutT>), dim3((uint32_t)nt),
+                     dim3(64 * NW), 0, s, planes, n, W, t0, (uint32_t)nt,
+                     out, cell_offset, max_dist, check_every);
+  HD_HIP(hipGetLastError());
+}
+
+template <int P, typename OutT>
+static void launch_allpairs_p(int v, const uint32_t *planes, uint32_t n,
+                              uint32_t n_pad, uint32_t W, uint64_t rb0,
+                              uint64_t rb1, OutT *out, uint64_t cell_offset,
+                              uint32_t max_dist, uint32_t check_every,
+                              hipStream_t s) {
+#define HD_V(idx, R, C, NW)                                                    \
+  case idx:                                                                    \
+    launch_allpairs_t<P, R, C, NW, OutT>(planes, n, n_pad, W, rb0, rb1, out,   \
+                                         cell_offset, max_dist, check_every, s); \
+    return;
+  switch (v) {
+    HD_V(0, 4, 16, 4)
+    HD_V(1, 8, 8, 4)
+    HD_V(2, 4, 8, 4)
+    HD_V(3, 2, 16, 4)
+    HD_V(4, 2, 8, 4)
+  default:
+    throw Error(HD_ERR_INVALID, "unknown tile variant");
+  }
+#undef HD_V
+}
+
+template <typename OutT>
+static void launch_allpairs_o(int P, int v, const uint32_t *planes, uint32_t n,
+                              uint32_t n_pad, uint32_t W, uint64_t rb0,
+                              uint64_t rb1, OutT *out, uint64_t cell_offset,
+                              uint32_t max_dist, uint32_t check_every,
+                              hipStream_t s) {
+  switch (P) {
+  case 4:
+    return launch_allpairs_p<4, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
+                                      cell_offset, max_dist, check_every, s);
+  case 5:
+    return launch_allpairs_p<5, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
+                                      cell_offset, max_dist, check_every, s);
+  case 6:
+    return launch_allpairs_p<6, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
+                                      cell_offset, max_dist, check_every, s);
+  default:
+    throw Error(HD_ERR_INVALID, "unsupported plane count");
+  }
+}
+
+// Geometry of one encoded dataset.
+struct Geometry {
+  uint64_t n = 0, bpr = 0, n_pad = 0;
+  uint32_t W = 0; // 32-site words per row
+  PlaneCode code;
+  uint64_t total_words() const { return (uint64_t)W * code.planes * n_pad; }
+};
+
+static constexpr uint64_t kRowPad = 512; // multiple of every variant's TI
+
+static Geometry geometry(uint64_t n, uint64_t bpr, int semantics, uint32_t mask) {
+  HD_REQUIRE(bpr > 0 && bpr % 8 == 0,
+             "bytes_per_seq must be a positive multiple of 8");
+  HD_REQUIRE(n < (1ull << 31), "too many sequences");
+  Geometry g;
+  g.n = n;
+  g.bpr = bpr;
+  g.n_pad = std::max<uint64_t>(kRowPad, (n + kRowPad - 1) / kRowPad * kRowPad);
+  g.W = (uint32_t)((2 * bpr + 31) / 32);
+  g.code = plane_code(semantics, mask);
+  return g;
+}
+
+static uint32_t tile_rows(int v) { return 64u * kVariants[v].R; }
+
+static void run_allpairs(const Geometry &g, const uint32_t *planes, int out_bytes,
+                         uint32_t max_dist, bool early_exit, int variant,
+                         uint64_t rb0, uint64_t rb1, void *out,
+                         uint64_t cell_offset, hipStream_t s) {
+  const uint32_t W = g.W;
+  // The early-exit vote runs every 32 words (1024 sites); it is skipped when
+  // max_dist is out of reach.
+  const uint32_t check_every =
+      (early_exit && (uint64_t)max_dist < 32ull * W) ? 32u : W;
+  const int P = g.code.planes;
+  switch (out_bytes) {
+  case 1:
+    launch_allpairs_o<uint8_t>(P, variant, planes, (uint32_t)g.n,
+                               (uint32_t)g.n_pad, W, rb0, rb1, (uint8_t *)out,
+                               cell_offset, max_dist, check_every, s);
+    break;
+  case 2:
+    launch_allpairs_o<uint16_t>(P, variant, planes, (uint32_t)g.n,
+                                (uint32_t)g.n_pad, W, rb0, rb1, (uint16_t *)out,
+                                cell_offset, max_dist, check_every, s);
+    break;
+  case 4:
+    launch_allpairs_o<uint32_t>(P, variant, planes, (uint32_t)g.n,
+                                (uint32_t)g.n_pad, W, rb0, rb1, (uint32_t *)out,
+                                cell_offset, max_dist, check_every, s);
+    break;
+  default:
+    throw Error(HD_ERR_INVALID, "out_bytes must be 1, 2 or 4");
+  }
+}
+
+static void check_max_dist(int out_bytes, uint32_t max_dist) {
+  HD_REQUIRE(out_bytes == 1 || out_bytes == 2 || out_bytes == 4,
+             "out_bytes must be 1, 2 or 4");
+  const uint64_t cap = out_bytes == 4 ? 0xFFFFFFFFull : ((1ull << (8 * out_bytes)) - 1);
+  HD_REQUIRE(max_dist <= cap, "max_dist does not fit the output cell type");
+}
+
+// ---------------------------------------------------------------------------
+// encode: packed nibble rows -> bit-sliced planes
+// ---------------------------------------------------------------------------
+// One thread per (row, run of 8 words): it reads 128 contiguous bytes of its
+// row and writes 8 x P plane words; the 64 rows of a wavefront form one row
+// group, so each store is 256 contiguous bytes.  Rows >= n and sites past the row end encode as '-' (all
+// planes set), which never counts against anything.
+static constexpr int kEncodeRun = 8;
+
+__global__ __launch_bounds__(256) void k_encode(const uint8_t *__restrict__ packed,
+                                                uint32_t n, uint64_t bpr,
+                                                uint32_t n_pad, uint32_t W,
+                                                int P, Table16 tbl,
+                                                uint32_t *__restrict__ planes) {
+  __shared__ uint8_t s_tbl[16];
+  if (threadIdx.x < 16) s_tbl[threadIdx.x] = tbl.v[threadIdx.x];
+  __syncthreads();
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pad) return;
+  const uint32_t k0 = blockIdx.y * kEncodeRun;
+  const uint8_t *row = packed + (uint64_t)i * bpr;
+  for (int kk = 0; kk < kEncodeRun; ++kk) {
+    const uint32_t k = k0 + kk;
+    if (k >= W) break;
+    uint32_t w[6] = {0, 0, 0, 0, 0, 0};
+    const uint64_t off = (uint64_t)k * 16;
+    uint32_t bytes[4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint64_t o = off + 8 * h;
+      if (i < n && o + 8 <= bpr) {
+        const uint2 u = *reinterpret_cast<const uint2 *>(row + o);
+        bytes[2 * h] = u.x;
+        bytes[2 * h + 1] = u.y;
+      } else {
+        bytes[2 * h] = 0xFFFFFFFFu;
+        bytes[2 * h + 1] = 0xFFFFFFFFu;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 32; ++s) {
+      const uint32_t nib = (bytes[s >> 3] >> (4 * (s & 7))) & 15u;
+      const uint32_t m = s_tbl[nib];
+#pragma unroll
+      for (int p = 0; p < 6; ++p)
+        if (p < P) w[p] |= ((m >> p) & 1u) << s;
+    }
+    uint32_t *dst = planes + (((uint64_t)(i / 64) * W + k) * P) * 64 + (i % 64);
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+      if (p < P) dst[p * 64] = w[p];
+  }
+}
+
+static void launch_encode(const Geometry &g, const uint8_t *d_packed,
+                          uint32_t *d_planes, hipStream_t s) {
+  Table16 t;
+  std::memcpy(t.v, g.code.table, 16);
+  const uint32_t runs = (g.W + kEncodeRun - 1) / kEncodeRun;
+  HD_REQUIRE(runs <= 65535, "sequences too long for the encode grid");
+  dim3 grid((uint32_t)((g.n_pad + 255) / 256), runs);
+  hipLaunchKernelGGL(k_encode, grid, dim3(256), 0, s, d_packed, (uint32_t)g.n,
+                     g.bpr, (uint32_t)g.n_pad, g.W, g.code.planes, t, d_planes);
+  HD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// nibble presence mask
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_nibble_mask(const uint2 *__restrict__ data,
+                                                     uint64_t n_units,
+                                                     uint32_t *__restrict__ mask) {
+  uint32_t m = 0;
+  for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n_units;
+       u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 v = data[u];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      m |= 1u << ((v.x >> (4 * s)) & 15u);
+      m |= 1u << ((v.y >> (4 * s)) & 15u);
+    }
+  }
+  // OR across the wavefront, then one atomic per wavefront.
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off);
+  if ((threadIdx.x & 63) == 0) atomicOr(mask, m);
+}
+
+static void launch_nibble_mask(const uint8_t *d_packed, uint64_t bytes,
+                               uint32_t *d_mask, hipStream_t s) {
+  const uint64_t units = bytes / 8;
+  HD_HIP(hipMemsetAsync(d_mask, 0, sizeof(uint32_t), s));
+  if (!units) return;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((units + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_nibble_mask, dim3(blocks), dim3(256), 0, s,
+                     reinterpret_cast<const uint2 *>(d_packed), units, d_mask);
+  HD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// consensus letter counts (input of the sparse/dense heuristic)
+// ---------------------------------------------------------------------------
+// Thread t owns the 4-byte word w (8 sites) of rows [r0, r0 + 512): the
+// loads of a wavefront are 256 contiguous bytes of one row.
+static constexpr uint32_t kCountRows = 512;
+
+__global__ __launch_bounds__(256) void k_letter_counts(const uint8_t *__restrict__ packed,
+                                                       uint32_t n, uint64_t bpr,
+                                                       uint64_t L,
+                                                       uint32_t *__restrict__ counts,
+                                                       uint32_t *__restrict__ mask) {
+  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t words = bpr / 4;
+  const uint32_t r0 = blockIdx.y * kCountRows;
+  const uint32_t r1 = min(n, r0 + kCountRows);
+  uint32_t c[8][4];
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+#pragma unroll
+    for (int l = 0; l < 4; ++l) c[s][l] = 0;
+  uint32_t m = 0;
+  if (w < words) {
+    for (uint32_t r = r0; r < r1; ++r) {
+      const uint32_t v = reinterpret_cast<const uint32_t *>(packed + (uint64_t)r * bpr)[w];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const uint32_t nib = (v >> (4 * s)) & 15u;
+        if (8 * w + s < L) m |= 1u << nib;
+        c[s][0] += nib == 1u;
+        c[s][1] += nib == 2u;
+        c[s][2] += nib == 4u;
+        c[s][3] += nib == 8u;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const uint64_t site = 8 * w + s;
+      if (site < L)
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+          if (c[s][l]) atomicAdd(&counts[site * 4 + l], c[s][l]);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off);
+  if ((threadIdx.x & 63) == 0 && m) atomicOr(mask, m);
+}
+
+// ---------------------------------------------------------------------------
+// reference distances: one query against many packed rows
+// ---------------------------------------------------------------------------
+// One wavefront per row; lane l reads 8-byte units l, l + 64, ... of the row
+// (512 contiguous bytes per load instruction) and of the query (L2 resident),
+// then the wavefront sums its 64 partial counts.
+template <int SEM>
+__device__ __forceinline__ uint32_t site_count32(uint32_t q, uint32_t r) {
+  if constexpr (SEM == HD_SEM_DENSE) {
+    // (q & r) nibble == 0: smear each nibble's bits into its bit 3.
+    uint32_t c = q & r;
+    c |= c << 1;
+    c |= c << 2;
+    return __builtin_popcount(~c & 0x88888888u);
+  } else {
+    // (q & r) == 0 || (q != r && q != '-' && r != '-')
+    uint32_t z = q & r;
+    z |= z << 1;
+    z |= z << 2; // bit 3: q & r != 0
+    uint32_t d = q ^ r;
+    d |= d << 1;
+    d |= d << 2; // bit 3: q != r
+    uint32_t gq = q & (q << 1);
+    gq &= gq << 2; // bit 3: q == '-'
+    uint32_t gr = r & (r << 1);
+    gr &= gr << 2; // bit 3: r == '-'
+    const uint32_t f = ~z | (d & ~(gq | gr));
+    return __builtin_popcount(f & 0x88888888u);
+  }
+}
+
+template <int SEM>
+__global__ __launch_bounds__(256) void k_refdist(const uint2 *__restrict__ query,
+                                                 const uint8_t *__restrict__ db,
+                                                 uint64_t bpr,
+                                                 const uint64_t *__restrict__ rows,
+                                                 uint64_t m,
+                                                 uint32_t *__restrict__ out) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t units = bpr / 8;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint64_t k = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+       k < m; k += waves) {
+    const uint64_t r = rows ? rows[k] : k;
+    const uint2 *row = reinterpret_cast<const uint2 *>(db + r * bpr);
+    uint32_t cnt = 0;
+    for (uint64_t u = lane; u < units; u += 64) {
+      const uint2 a = row[u];
+      const uint2 q = query[u];
+      cnt += site_count32<SEM>(q.x, a.x) + site_count32<SEM>(q.y, a.y);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if (lane == 0) out[k] = cnt;
+  }
+}
+
+static void launch_refdist(const uint8_t *d_query, const uint8_t *d_db,
+                           uint64_t bpr, const uint64_t *d_rows, uint64_t m,
+                           int semantics, uint32_t *d_out, hipStream_t s) {
+  if (m == 0) return;
+  const uint64_t wpb = 4; // wavefronts per block
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((m + wpb - 1) / wpb, 1u << 16);
+  if (semantics == HD_SEM_DENSE)
+    hipLaunchKernelGGL(k_refdist<HD_SEM_DENSE>, dim3(blocks), dim3(64 * wpb), 0, s,
+                       reinterpret_cast<const uint2 *>(d_query), d_db, bpr, d_rows,
+                       m, d_out);
+  else if (semantics == HD_SEM_CHAR_X)
+    hipLaunchKernelGGL(k_refdist<HD_SEM_CHAR_X>, dim3(blocks), dim3(64 * wpb), 0, s,
+                       reinterpret_cast<const uint2 *>(d_query), d_db, bpr, d_rows,
+                       m, d_out);
+  else
+    throw Error(HD_ERR_INVALID,
+                "reference distances support HD_SEM_DENSE and HD_SEM_CHAR_X");
+  HD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// sparse emit: cells <= threshold of a device triangle, row-major order
+// ---------------------------------------------------------------------------
+// Pass 1 counts qualifying cells per block of kEmitCells cells; an exclusive
+// scan of the counts gives each block its output offset; pass 2 writes the
+// triplets of a block in cell order using wavefront ballots.
+static constexpr uint32_t kEmitCells = 4096; // per 256-thread block
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_emit_count(const T *__restrict__ cells,
+                                                    uint64_t n_cells,
+                                                    uint32_t thr,
+                                                    uint32_t *__restrict__ counts) {
+  const uint64_t base = (uint64_t)blockIdx.x * kEmitCells;
+  uint32_t c = 0;
+  for (uint32_t o = threadIdx.x; o < kEmitCells; o += 256) {
+    const uint64_t x = base + o;
+    if (x < n_cells && (uint32_t)cells[x] <= thr) ++c;
+  }
+  __shared__ uint32_t s_sum[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) counts[blockIdx.x] = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
+}
+
+__device__ __forceinline__ uint64_t row_of_cell(uint64_t x) {
+  // Largest i with i (i - 1) / 2 <= x.
+  uint64_t i = (uint64_t)floor(sqrt(2.0 * (double)x + 0.25) + 0.5);
+  while (i * (i - 1) / 2 > x) --i;
+  while ((i + 1) * i / 2 <= x) ++i;
+  return i;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_emit_write(const T *__restrict__ cells,
+                                                    uint64_t n_cells, uint32_t thr,
+                                                    const uint64_t *__restrict__ offsets,
+                                                    uint64_t capacity,
+                                                    uint32_t *__restrict__ trip) {
+  const uint64_t base = (uint64_t)blockIdx.x * kEmitCells;
+  __shared__ uint32_t s_wave[4];
+  __shared__ uint64_t s_run;
+  if (threadIdx.x == 0) s_run = offsets[blockIdx.x];
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t o = 0; o < kEmitCells; o += 256) {
+    const uint64_t x = base + o + threadIdx.x;
+    const bool hit = x < n_cells && (uint32_t)cells[x] <= thr;
+    const uint64_t bal = __ballot(hit);
+    const uint32_t before = __popcll(bal & ((1ull << lane) - 1));
+    if (lane == 0) s_wave[wave] = __popcll(bal);
+    __syncthreads();
+    uint64_t pos = s_run;
+    for (uint32_t w = 0; w < wave; ++w) pos += s_wave[w];
+    pos += before;
+    if (hit && pos < capacity) {
+      const uint64_t i = row_of_cell(x);
+      const uint64_t j = x - i * (i - 1) / 2;
+      trip[3 * pos + 0] = (uint32_t)i;
+      trip[3 * pos + 1] = (uint32_t)j;
+      trip[3 * pos + 2] = (uint32_t)cells[x];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_run += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    __syncthreads();
+  }
+}
+
+// Emits into device triplets; returns the total number of qualifying cells.
+template <typename T>
+static uint64_t emit_t(const T *d_cells, uint64_t n_cells, uint32_t thr,
+                       uint32_t *d_trip, uint64_t capacity, hipStream_t s) {
+  const uint64_t blocks = (n_cells + kEmitCells - 1) / kEmitCells;
+  if (!blocks) return 0;
+  HD_REQUIRE(blocks < (1ull << 31), "triangle too large for the emit grid");
+  DeviceBuffer d_counts(blocks * sizeof(uint32_t));
+  hipLaunchKernelGGL((k_emit_count<T>), dim3((uint32_t)blocks), dim3(256), 0, s,
+                     d_cells, n_cells, thr, d_counts.get<uint32_t>());
+  HD_HIP(hipGetLastError());
+  std::vector<uint32_t> counts(blocks);
+  HD_HIP(hipMemcpyAsync(counts.data(), d_counts.get(), blocks * 4,
+                        hipMemcpyDeviceToHost, s));
+  HD_HIP(hipStreamSynchronize(s));
+  std::vector<uint64_t> offsets(blocks);
+  uint64_t total = 0;
+  for (uint64_t b = 0; b < blocks; ++b) {
+    offsets[b] = total;
+    total += counts[b];
+  }
+  if (total && capacity) {
+    DeviceBuffer d_off(blocks * sizeof(uint64_t));
+    HD_HIP(hipMemcpyAsync(d_off.get(), offsets.data(), blocks * 8,
+                          hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((k_emit_write<T>), dim3((uint32_t)blocks), dim3(256), 0, s,
+                       d_cells, n_cells, thr, d_off.get<uint64_t>(), capacity, d_trip);
+    HD_HIP(hipGetLastError());
+    HD_HIP(hipStreamSynchronize(s));
+  }
+  return total;
+}
+
+// ---------------------------------------------------------------------------
+// library stream
+// ---------------------------------------------------------------------------
+static hipStream_t lib_stream() {
+  static std::once_flag once;
+  static hipStream_t s = nullptr;
+  std::call_once(once, [] {
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+  });
+  if (!s) throw Error(HD_ERR_HIP, "could not create the library stream");
+  return s;
+}
+
+// Uploads n packed rows and builds their planes (plus the nibble mask).
+struct Encoded {
+  Geometry g;
+  DeviceBuffer packed, planes, mask;
+};
+
+static void upload_encode(Encoded &e, const uint8_t *packed, uint64_t n,
+                          uint64_t bpr, int semantics, hipStream_t s) {
+  geometry(n, bpr, HD_SEM_DENSE, 0); // argument checks
+  e.packed.alloc(n * bpr);
+  HD_HIP(hipMemcpyAsync(e.packed.get(), packed, n * bpr, hipMemcpyHostToDevice, s));
+  uint32_t mask = 0xFFFFu;
+  if (semantics != HD_SEM_DENSE) {
+    e.mask.alloc(sizeof(uint32_t));
+    launch_nibble_mask(e.packed.get<uint8_t>(), n * bpr, e.mask.get<uint32_t>(), s);
+    HD_HIP(hipMemcpyAsync(&mask, e.mask.get(), 4, hipMemcpyDeviceToHost, s));
+    HD_HIP(hipStreamSynchronize(s));
+  }
+  e.g = geometry(n, bpr, semantics, mask);
+  e.planes.alloc(e.g.total_words() * 4);
+  launch_encode(e.g, e.packed.get<uint8_t>(), e.planes.get<uint32_t>(), s);
+}
+
+} // namespace hd
+
+using namespace hd;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *hd_last_error(void) { return g_last_error.c_str(); }
+
+int hd_device_count(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+  return count;
+}
+
+int hd_device_arch(char *buf, size_t buf_len) {
+  return guarded([&] {
+    ensure_device();
+    HD_REQUIRE(buf && buf_len > 0, "buffer required");
+    hipDeviceProp_t prop;
+    HD_HIP(hipGetDeviceProperties(&prop, 0));
+    std::snprintf(buf, buf_len, "%s", prop.gcnArchName);
+  });
+}
+
+int hd_variant_count(void) { return kNumVariants; }
+
+int hd_variant_info(int variant, int *rows_per_lane, int *cols_per_wave,
+                    int *waves, int *planes) {
+  return guarded([&] {
+    HD_REQUIRE(variant >= 0 && variant < kNumVariants, "unknown variant");
+    if (rows_per_lane) *rows_per_lane = kVariants[variant].R;
+    if (cols_per_wave) *cols_per_wave = kVariants[variant].C;
+    if (waves) *waves = kVariants[variant].NW;
+    if (planes) *planes = 0;
+  });
+}
+
+int hd_planes_layout(uint64_t n, uint64_t bytes_per_seq, int semantics,
+                     uint32_t nibble_mask, uint64_t *n_pad, uint32_t *n_words,
+                     uint32_t *n_planes, uint64_t *total_words) {
+  return guarded([&] {
+    const Geometry g = geometry(n, bytes_per_seq, semantics, nibble_mask);
+    if (n_pad) *n_pad = g.n_pad;
+    if (n_words) *n_words = g.W;
+    if (n_planes) *n_planes = (uint32_t)g.code.planes;
+    if (total_words) *total_words = g.total_words();
+  });
+}
+
+int hd_nibble_mask_dev(const uint8_t *d_packed, uint64_t n, uint64_t bytes_per_seq,
+                       uint32_t *mask, void *stream) {
+  return guarded([&] {
+    ensure_device();
+    HD_REQUIRE(mask, "mask pointer required");
+    hipStream_t s = (hipStream_t)stream;
+    DeviceBuffer d_mask(4);
+    launch_nibble_mask(d_packed, n * bytes_per_seq, d_mask.get<uint32_t>(), s);
+    HD_HIP(hipMemcpyAsync(mask, d_mask.get(), 4, hipMemcpyDeviceToHost, s));
+    HD_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int hd_encode_dev(const uint8_t *d_packed, uint64_t n, uint64_t bytes_per_seq,
+                  int semantics, uint32_t nibble_mask, uint32_t *d_planes,
+                  void *stream) {
+  return guarded([&] {
+    ensure_device();
+    const Geometry g = geometry(n, bytes_per_seq, semantics, nibble_mask);
+    launch_encode(g, d_packed, d_planes, (hipStream_t)stream);
+  });
+}
+
+int hd_all_pairs_dev(const uint32_t *d_planes, uint64_t n, uint64_t bytes_per_seq,
+                     int semantics, uint32_t nibble_mask, int out_bytes,
+                     uint32_t max_dist, int early_exit, int variant, void *d_out,
+                     void *stream) {
+  return guarded([&] {
+    ensure_device();
+    check_max_dist(out_bytes, max_dist);
+    const Geometry g = geometry(n, bytes_per_seq, semantics, nibble_mask);
+    if (n < 2) return;
+    const int v = variant < 0 ? default_variant(g.code.planes, n) : variant;
+    HD_REQUIRE(v < kNumVariants, "unknown variant");
+    const uint64_t TI = tile_rows(v);
+    const uint64_t nb = (n + TI - 1) / TI;
+    run_allpairs(g, d_planes, out_bytes, max_dist, early_exit != 0, v, 0, nb,
+                 d_out, 0, (hipStream_t)stream);
+  });
+}
+
+int hd_all_pairs(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
+                 int semantics, int out_bytes, uint32_t max_dist, void *host_out) {
+  return guarded([&] {
+    check_max_dist(out_bytes, max_dist);
+    if (n < 2) return;
+    HD_REQUIRE(packed && host_out, "null buffer");
+    ensure_device();
+    hipStream_t s = lib_stream();
+    Encoded e;
+    upload_encode(e, packed, n, bytes_per_seq, semantics, s);
+    const uint64_t cells = tri_cells(n);
+    DeviceBuffer d_out(cells * out_bytes);
+    const int v = default_variant(e.g.code.planes, n);
+    const uint64_t TI = tile_rows(v);
+    run_allpairs(e.g, e.planes.get<uint32_t>(), out_bytes, max_dist, true, v, 0,
+                 (n + TI - 1) / TI, d_out.get(), 0, s);
+    HD_HIP(hipMemcpyAsync(host_out, d_out.get(), cells * out_bytes,
+                          hipMemcpyDeviceToHost, s));
+    HD_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int hd_all_pairs_stream(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
+                        int semantics, int out_bytes, uint32_t max_dist,
+                        uint64_t band_cells, hd_band_cb cb, void *user) {
+  return guarded([&] {
+    check_max_dist(out_bytes, max_dist);
+    HD_REQUIRE(cb, "callback required");
+    if (n < 2) return;
+    HD_REQUIRE(packed, "null buffer");
+    ensure_device();
+    hipStream_t s = lib_stream();
+    Stream copy;
+    Encoded e;
+    upload_encode(e, packed, n, bytes_per_seq, semantics, s);
+    const int v = default_variant(e.g.code.planes, n);
+    const uint64_t TI = tile_rows(v);
+    const uint64_t nb = (n + TI - 1) / TI;
+    if (band_cells == 0) band_cells = 64ull << 20;
+
+    // Bands of whole row blocks, each at least one block.
+    std::vector<uint64_t> band_rb{0};
+    while (band_rb.back() < nb) {
+      uint64_t rb = band_rb.back() + 1;
+      const uint64_t r0 = band_rb.back() * TI;
+      while (rb < nb && tri_cells(std::min(n, (rb + 1) * TI)) - tri_cells(r0) <= band_cells)
+        ++rb;
+      band_rb.push_back(rb);
+    }
+    const size_t n_bands = band_rb.size() - 1;
+    uint64_t max_band = 0;
+    for (size_t b = 0; b < n_bands; ++b)
+      max_band = std::max(max_band, tri_cells(std::min(n, band_rb[b + 1] * TI)) -
+                                        tri_cells(band_rb[b] * TI));
+    const uint64_t bytes = max_band * out_bytes;
+    DeviceBuffer dbuf[2];
+    PinnedBuffer hbuf[2];
+    Event ev_k[2], ev_c[2];
+    for (int i = 0; i < 2; ++i) {
+      dbuf[i].alloc(bytes);
+      hbuf[i].alloc(bytes);
+    }
+    auto band_range = [&](size_t b, uint64_t &r0, uint64_t &r1) {
+      r0 = band_rb[b] * TI;
+      r1 = std::min(n, band_rb[b + 1] * TI);
+    };
+    auto enqueue = [&](size_t b) {
+      uint64_t r0, r1;
+      band_range(b, r0, r1);
+      const int slot = (int)(b & 1);
+      run_allpairs(e.g, e.planes.get<uint32_t>(), out_bytes, max_dist, true, v,
+                   band_rb[b], band_rb[b + 1], dbuf[slot].get(), tri_cells(r0), s);
+      HD_HIP(hipEventRecord(ev_k[slot].get(), s));
+      HD_HIP(hipStreamWaitEvent(copy.get(), ev_k[slot].get(), 0));
+      HD_HIP(hipMemcpyAsync(hbuf[slot].get(), dbuf[slot].get(),
+                            (tri_cells(r1) - tri_cells(r0)) * out_bytes,
+                            hipMemcpyDeviceToHost, copy.get()));
+      HD_HIP(hipEventRecord(ev_c[slot].get(), copy.get()));
+    };
+    for (size_t b = 0; b < std::min<size_t>(2, n_bands); ++b) enqueue(b);
+    for (size_t b = 0; b < n_bands; ++b) {
+      const int slot = (int)(b & 1);
+      HD_HIP(hipEventSynchronize(ev_c[slot].get()));
+      uint64_t r0, r1;
+      band_range(b, r0, r1);
+      const uint64_t c0 = tri_cells(r0), nc = tri_cells(r1) - c0;
+      if (nc && cb(user, hbuf[slot].get(), c0, nc, r0 == 0 ? 1 : r0, r1) != 0) {
+        HD_HIP(hipStreamSynchronize(s));
+        HD_HIP(hipStreamSynchronize(copy.get()));
+        throw Error(HD_ERR_CALLBACK, "band callback aborted the stream");
+      }
+      if (b + 2 < n_bands) enqueue(b + 2);
+    }
+    HD_HIP(hipStreamSynchronize(s));
+    HD_HIP(hipStreamSynchronize(copy.get()));
+  });
+}
+
+int hd_distance_pair(const uint8_t *a, const uint8_t *b, uint64_t bytes,
+                     int semantics, int32_t max_dist, int32_t *out) {
+  return guarded([&] {
+    HD_REQUIRE(a && b && out, "null buffer");
+    HD_REQUIRE(max_dist >= 0, "max_dist must be >= 0");
+    std::vector<uint8_t> both(2 * bytes);
+    std::memcpy(both.data(), a, bytes);
+    std::memcpy(both.data() + bytes, b, bytes);
+    uint32_t d = 0;
+    const int st = hd_all_pairs(both.data(), 2, bytes, semantics, 4,
+                                (uint32_t)max_dist, &d);
+    if (st != HD_OK) throw Error(st, g_last_error);
+    *out = (int32_t)d;
+  });
+}
+
+int hd_reference_distances_dev(const uint8_t *d_query, const uint8_t *d_db,
+                               uint64_t n_db, uint64_t bytes_per_seq,
+                               const uint64_t *d_rows, uint64_t m, int semantics,
+                               uint32_t *d_out, void *stream) {
+  return guarded([&] {
+    ensure_device();
+    HD_REQUIRE(bytes_per_seq > 0 && bytes_per_seq % 8 == 0,
+               "bytes_per_seq must be a positive multiple of 8");
+    HD_REQUIRE(d_rows || m <= n_db, "m exceeds the database rows");
+    launch_refdist(d_query, d_db, bytes_per_seq, d_rows, m, semantics, d_out,
+                   (hipStream_t)stream);
+  });
+}
+
+int hd_reference_distances(const uint8_t *query, const uint8_t *db, uint64_t n_db,
+                           uint64_t bytes_per_seq, const uint64_t *rows, uint64_t m,
+                           int semantics, uint32_t *out) {
+  return guarded([&] {
+    HD_REQUIRE(bytes_per_seq > 0 && bytes_per_seq % 8 == 0,
+               "bytes_per_seq must be a positive multiple of 8");
+    HD_REQUIRE(semantics == HD_SEM_DENSE || semantics == HD_SEM_CHAR_X,
+               "reference distances support HD_SEM_DENSE and HD_SEM_CHAR_X");
+    if (m == 0) return;
+    HD_REQUIRE(query && db && out, "null buffer");
+    if (rows) {
+      for (uint64_t k = 0; k < m; ++k)
+        HD_REQUIRE(rows[k] < n_db, "row index out of range");
+    } else {
+      HD_REQUIRE(m <= n_db, "m exceeds the database rows");
+    }
+    ensure_device();
+    hipStream_t s = lib_stream();
+    DeviceBuffer d_q(bytes_per_seq), d_db(n_db * bytes_per_seq), d_out(m * 4), d_rows;
+    HD_HIP(hipMemcpyAsync(d_q.get(), query, bytes_per_seq, hipMemcpyHostToDevice, s));
+    HD_HIP(hipMemcpyAsync(d_db.get(), db, n_db * bytes_per_seq, hipMemcpyHostToDevice, s));
+    if (rows) {
+      d_rows.alloc(m * 8);
+      HD_HIP(hipMemcpyAsync(d_rows.get(), rows, m * 8, hipMemcpyHostToDevice, s));
+    }
+    launch_refdist(d_q.get<uint8_t>(), d_db.get<uint8_t>(), bytes_per_seq,
+                   rows ? d_rows.get<uint64_t>() : nullptr, m, semantics,
+                   d_out.get<uint32_t>(), s);
+    HD_HIP(hipMemcpyAsync(out, d_out.get(), m * 4, hipMemcpyDeviceToHost, s));
+    HD_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int hd_consensus_diff(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
+                      uint64_t L, uint64_t *n_diff, uint32_t *nibble_mask) {
+  return guarded([&] {
+    HD_REQUIRE(bytes_per_seq > 0 && bytes_per_seq % 8 == 0,
+               "bytes_per_seq must be a positive multiple of 8");
+    HD_REQUIRE(L <= 2 * bytes_per_seq, "L exceeds the packed row length");
+    HD_REQUIRE(n_diff && nibble_mask, "null output");
+    *n_diff = 0;
+    *nibble_mask = 0;
+    if (n == 0 || L == 0) return;
+    HD_REQUIRE(packed, "null buffer");
+    HD_REQUIRE(n < (1ull << 32), "too many sequences");
+    ensure_device();
+    hipStream_t s = lib_stream();
+    DeviceBuffer d_packed(n * bytes_per_seq), d_counts(L * 16), d_mask(4);
+    HD_HIP(hipMemcpyAsync(d_packed.get(), packed, n * bytes_per_seq,
+                          hipMemcpyHostToDevice, s));
+    HD_HIP(hipMemsetAsync(d_counts.get(), 0, L * 16, s));
+    HD_HIP(hipMemsetAsync(d_mask.get(), 0, 4, s));
+    const uint64_t words = bytes_per_seq / 4;
+    dim3 grid((uint32_t)((words + 255) / 256), (uint32_t)((n + kCountRows - 1) / kCountRows));
+    HD_REQUIRE(grid.y <= 65535, "too many sequences for the count grid");
+    hipLaunchKernelGGL(k_letter_counts, grid, dim3(256), 0, s,
+                       d_packed.get<uint8_t>(), (uint32_t)n, bytes_per_seq, L,
+                       d_counts.get<uint32_t>(), d_mask.get<uint32_t>());
+    HD_HIP(hipGetLastError());
+    std::vector<uint32_t> counts(L * 4);
+    HD_HIP(hipMemcpyAsync(counts.data(), d_counts.get(), L * 16, hipMemcpyDeviceToHost, s));
+    HD_HIP(hipMemcpyAsync(nibble_mask, d_mask.get(), 4, hipMemcpyDeviceToHost, s));
+    HD_HIP(hipStreamSynchronize(s));
+    // Consensus letter = first maximum over A, C, G, T; every other entry of
+    // the column differs from it (src/hamming_impl.cc:157-166, 177-185).
+    uint64_t diff = 0;
+    for (uint64_t site = 0; site < L; ++site) {
+      const uint32_t *c = &counts[site * 4];
+      uint32_t mx = c[0];
+      for (int l = 1; l < 4; ++l) mx = std::max(mx, c[l]);
+      diff += n - mx;
+    }
+    *n_diff = diff;
+  });
+}
+
+int hd_sparse_emit(const void *cells, uint64_t n, int out_bytes, int64_t threshold,
+                   uint32_t *triplets, uint64_t capacity, uint64_t *count) {
+  return guarded([&] {
+    HD_REQUIRE(count, "null count");
+    HD_REQUIRE(out_bytes == 1 || out_bytes == 2 || out_bytes == 4,
+               "out_bytes must be 1, 2 or 4");
+    *count = 0;
+    const uint64_t nc = tri_cells(n);
+    if (nc == 0 || threshold < 0) return;
+    HD_REQUIRE(cells, "null buffer");
+    HD_REQUIRE(triplets || capacity == 0, "null triplets");
+    ensure_device();
+    const uint32_t thr = threshold > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)threshold;
+    hipStream_t s = lib_stream();
+    DeviceBuffer d_cells(nc * out_bytes);
+    HD_HIP(hipMemcpyAsync(d_cells.get(), cells, nc * out_bytes, hipMemcpyHostToDevice, s));
+    DeviceBuffer d_trip(std::max<uint64_t>(capacity, 1) * 12);
+    uint64_t total = 0;
+    if (out_bytes == 1)
+      total = emit_t(d_cells.get<uint8_t>(), nc, thr, d_trip.get<uint32_t>(), capacity, s);
+    else if (out_bytes == 2)
+      total = emit_t(d_cells.get<uint16_t>(), nc, thr, d_trip.get<uint32_t>(), capacity, s);
+    else
+      total = emit_t(d_cells.get<uint32_t>(), nc, thr, d_trip.get<uint32_t>(), capacity, s);
+    const uint64_t got = std::min(total, capacity);
+    if (got) {
+      HD_HIP(hipMemcpyAsync(triplets, d_trip.get(), got * 12, hipMemcpyDeviceToHost, s));
+      HD_HIP(hipStreamSynchronize(s));
+    }
+    *count = total;
+    if (total > capacity)
+      throw Error(HD_ERR_CAPACITY, "more cells below the threshold than capacity");
+  });
+}
+
+} // extern "C"

@@ -1,0 +1,146 @@
+// Shared internals of the HIP engine: error reporting, HIP call checking,
+// RAII device/pinned buffers and the plane encoding chosen per semantics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "hammingdist_amd.h"
+
+namespace hd {
+
+// Error carried from an internal failure to the C ABI status code.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &msg) : std::runtime_error(msg), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define HD_HIP(call)                                                           \
+  do {                                                                         \
+    hipError_t hd_e_ = (call);                                                 \
+    if (hd_e_ != hipSuccess)                                                   \
+      throw ::hd::Error(HD_ERR_HIP, std::string(#call) + ": " +               \
+                                        hipGetErrorString(hd_e_));            \
+  } while (0)
+
+#define HD_REQUIRE(cond, msg)                                                  \
+  do {                                                                         \
+    if (!(cond))                                                               \
+      throw ::hd::Error(HD_ERR_INVALID, (msg));                                \
+  } while (0)
+
+// Runs f, converting exceptions into a status code plus hd_last_error text.
+template <typename F> int guarded(F &&f) {
+  try {
+    f();
+    return HD_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return HD_ERR_CAPACITY;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return HD_ERR_INVALID;
+  }
+}
+
+void ensure_device();
+
+// Device buffer owned for the duration of one call.
+class DeviceBuffer {
+public:
+  DeviceBuffer() = default;
+  explicit DeviceBuffer(size_t bytes) { alloc(bytes); }
+  ~DeviceBuffer() { release(); }
+  DeviceBuffer(const DeviceBuffer &) = delete;
+  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+  void alloc(size_t bytes) {
+    release();
+    if (bytes) HD_HIP(hipMalloc(&p_, bytes));
+    n_ = bytes;
+  }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  template <typename T = void> T *get() const { return static_cast<T *>(p_); }
+  size_t size() const { return n_; }
+
+private:
+  void *p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// Pinned host buffer (hipHostMalloc) for asynchronous copies.
+class PinnedBuffer {
+public:
+  PinnedBuffer() = default;
+  explicit PinnedBuffer(size_t bytes) { alloc(bytes); }
+  ~PinnedBuffer() { release(); }
+  PinnedBuffer(const PinnedBuffer &) = delete;
+  PinnedBuffer &operator=(const PinnedBuffer &) = delete;
+  void alloc(size_t bytes) {
+    release();
+    if (bytes) HD_HIP(hipHostMalloc(&p_, bytes, hipHostMallocDefault));
+    n_ = bytes;
+  }
+  void release() {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  template <typename T = void> T *get() const { return static_cast<T *>(p_); }
+  size_t size() const { return n_; }
+
+private:
+  void *p_ = nullptr;
+  size_t n_ = 0;
+};
+
+class Stream {
+public:
+  Stream() { HD_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking)); }
+  ~Stream() { (void)hipStreamDestroy(s_); }
+  Stream(const Stream &) = delete;
+  Stream &operator=(const Stream &) = delete;
+  hipStream_t get() const { return s_; }
+
+private:
+  hipStream_t s_{};
+};
+
+class Event {
+public:
+  Event() { HD_HIP(hipEventCreateWithFlags(&e_, hipEventDisableTiming)); }
+  ~Event() { (void)hipEventDestroy(e_); }
+  Event(const Event &) = delete;
+  Event &operator=(const Event &) = delete;
+  hipEvent_t get() const { return e_; }
+
+private:
+  hipEvent_t e_{};
+};
+
+// Bit-sliced plane encoding of one dataset: every nibble code maps to a
+// P-bit mask, one bit per plane.  A site pair counts as distance 1 exactly
+// when no plane has the bit set in both sequences (sum of AND over planes is
+// zero), which is how every semantics of the engine is evaluated.
+struct PlaneCode {
+  int planes = 4;
+  uint8_t table[16] = {};
+};
+
+PlaneCode plane_code(int semantics, uint32_t nibble_mask);
+
+inline uint64_t tri_cells(uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; }
+
+} // namespace hd

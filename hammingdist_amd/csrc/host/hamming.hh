@@ -1,0 +1,125 @@
+// Host layer of hammingdist_amd: the reference's C++ library interface
+// (include/hamming/hamming.hh, hamming_impl.hh, hamming_utils.hh) on top of the
+// HIP engine's C ABI (include/hammingdist_amd.h).  Parsing, packing, result
+// containers and text output live here; every distance is computed by the
+// engine on the GPU.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "hammingdist_amd.h"
+
+namespace hdhost {
+
+using GeneBlock = std::uint8_t;
+using ReferenceDistIntType = std::uint32_t;
+using DefaultDistIntType = std::uint8_t;
+
+// ---- encoding (src/hamming_impl.cc:23-44, 244-267) ------------------------
+std::array<GeneBlock, 256> lookup_table(bool include_x = false);
+// Bytes of one packed row of L sites: two sites per byte, padded to 8 bytes.
+std::uint64_t bytes_per_seq(std::uint64_t L);
+// Packs one sequence (from_string) into bps bytes; bytes past the sequence
+// are 0xFF ('-' in both nibbles).
+void pack_row(const char *s, std::uint64_t L, const std::array<GeneBlock, 256> &lut,
+              GeneBlock *out, std::uint64_t bps);
+// Packs every sequence into n * bps contiguous bytes (parallel over rows).
+std::vector<GeneBlock> pack_all(const std::vector<std::string> &data, bool include_x,
+                                std::uint64_t bps);
+
+// ---- FASTA ingest (src/hamming_impl.cc:80-91, 200-242; src/hamming.cc:76-102)
+struct FastaData {
+  std::vector<std::string> data;
+  std::vector<std::size_t> sequence_indices;
+};
+FastaData read_fasta(const std::string &filename, bool remove_duplicates = false,
+                     std::size_t n = 0);
+std::vector<std::size_t> fasta_sequence_indices(const std::string &fasta_file,
+                                                std::size_t n = 0);
+void validate_data(const std::vector<std::string> &data);
+
+// ---- engine bridge --------------------------------------------------------
+bool gpu_available();
+// Throws std::runtime_error with the engine's message when status != HD_OK.
+void check(int status);
+
+// safe_int_cast<T>(x), include/hamming/hamming_impl.hh:37-45
+template <typename T> T safe_int_cast(int x, T max_x = std::numeric_limits<T>::max()) {
+  if (x > static_cast<int>(max_x)) return max_x;
+  return static_cast<T>(x);
+}
+
+// Route of distances<T>: which per-site semantics the reference would use
+// for these inputs (include/hamming/hamming_impl.hh:95-118, 147-159) and the
+// matching packing.  Throws the reference's errors for invalid flag mixes.
+struct Route {
+  int semantics = HD_SEM_DENSE;
+  bool pack_include_x = false;
+  bool sparse = false;
+};
+Route choose_route(const std::vector<std::string> &data, bool include_x, bool use_gpu,
+                   std::vector<GeneBlock> &packed, std::uint64_t bps);
+
+// distances<T>, include/hamming/hamming_impl.hh:68-176
+template <typename T>
+std::vector<T> distances(std::vector<std::string> &data, bool include_x,
+                         bool clear_input_data, bool use_gpu, int max_distance);
+
+// ---- result container (include/hamming/hamming.hh:22-143) -----------------
+template <typename T> struct DataSet {
+  explicit DataSet(std::vector<std::string> &data, bool include_x = false,
+                   bool clear_input_data = false, std::vector<std::size_t> &&indices = {},
+                   bool use_gpu = false,
+                   int max_distance = std::numeric_limits<int>::max());
+  // Full-matrix CSV reader (from_csv), include/hamming/hamming.hh:34-55.
+  explicit DataSet(const std::string &filename);
+  explicit DataSet(std::vector<T> &&distances);
+
+  void dump(const std::string &filename) const;
+  void dump_lower_triangular(const std::string &filename) const;
+  void dump_sparse(const std::string &filename, int threshold) const;
+  void dump_sequence_indices(const std::string &filename) const;
+  int get(std::size_t i, std::size_t j) const;
+
+  std::size_t nsamples = 0;
+  std::vector<T> result;
+  std::vector<std::size_t> sequence_indices{};
+};
+
+DataSet<DefaultDistIntType> from_stringlist(std::vector<std::string> &data, bool include_x,
+                                            bool use_gpu, int max_distance);
+DataSet<DefaultDistIntType> from_csv(const std::string &filename);
+template <typename T> DataSet<T> from_lower_triangular(const std::string &filename);
+template <typename T>
+DataSet<T> from_fasta(const std::string &filename, bool include_x, bool remove_duplicates,
+                      std::size_t n, bool use_gpu, int max_distance);
+void from_fasta_to_lower_triangular(const std::string &input_filename,
+                                    const std::string &output_filename,
+                                    bool remove_duplicates, std::size_t n, bool use_gpu,
+                                    int max_distance);
+ReferenceDistIntType distance(const std::string &seq0, const std::string &seq1,
+                              bool include_x = false);
+std::vector<ReferenceDistIntType> fasta_reference_distances(
+    const std::string &reference_sequence, const std::string &fasta_file,
+    bool include_x = false);
+
+// ---- text output (include/hamming/hamming_utils.hh:14-110) ----------------
+// Writes whole rows [row_begin, row_end) of a triangle whose first cell in
+// `cells` is (row_begin, 0); truncates the file when `append` is false.
+template <typename T>
+void write_rows(const std::string &filename, const T *cells, std::uint64_t row_begin,
+                std::uint64_t row_end, bool append);
+template <typename T>
+void write_lower_triangular(const std::string &filename, const std::vector<T> &cells);
+
+// ---- small parallel helpers -----------------------------------------------
+unsigned worker_count();
+template <typename F> void parallel_for(std::uint64_t n, F &&f);
+
+} // namespace hdhost
+
+#include "parallel.hh"

@@ -1,0 +1,198 @@
+// Text output of the host layer: lower-triangular rows and sparse triplets.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <stdexcept>
+
+#include "hamming.hh"
+
+namespace hdhost {
+
+namespace {
+
+// Decimal text of 0..65535: digits and length, built once.
+struct DecimalTable {
+  char digits[65536][5];
+  std::uint8_t len[65536];
+  DecimalTable() {
+    for (unsigned v = 0; v < 65536; ++v) {
+      char tmp[8];
+      const int l = std::snprintf(tmp, sizeof tmp, "%u", v);
+      len[v] = static_cast<std::uint8_t>(l);
+      std::memcpy(digits[v], tmp, static_cast<std::size_t>(l));
+    }
+  }
+};
+
+const DecimalTable &decimals() {
+  static const DecimalTable t;
+  return t;
+}
+
+inline char *put_uint(char *p, std::uint32_t v) {
+  if (v < 65536) {
+    const auto &t = decimals();
+    std::memcpy(p, t.digits[v], 5);
+    return p + t.len[v];
+  }
+  char tmp[16];
+  const int l = std::snprintf(tmp, sizeof tmp, "%u", v);
+  std::memcpy(p, tmp, static_cast<std::size_t>(l));
+  return p + l;
+}
+
+constexpr std::size_t kMaxDigits = 10;
+
+// Rows [r0, r1) as text; `cells` points at cell (r0, 0).  Row i is
+// "d(i,0),...,d(i,i-1)\n" (include/hamming/hamming_utils.hh:19-53).
+template <typename T>
+std::size_t format_rows(const T *cells, std::uint64_t r0, std::uint64_t r1, char *out) {
+  char *p = out;
+  const T *c = cells;
+  for (std::uint64_t i = r0; i < r1; ++i) {
+    for (std::uint64_t j = 0; j < i; ++j) {
+      p = put_uint(p, static_cast<std::uint32_t>(*c++));
+      *p++ = ',';
+    }
+    if (i > 0) p[-1] = '\n';
+  }
+  return static_cast<std::size_t>(p - out);
+}
+
+void write_all(std::FILE *f, const char *data, std::size_t len) {
+  if (len && std::fwrite(data, 1, len, f) != len)
+    throw std::runtime_error("write to lower-triangular file failed");
+}
+
+} // namespace
+
+template <typename T>
+void write_rows(const std::string &filename, const T *cells, std::uint64_t row_begin,
+                std::uint64_t row_end, bool append) {
+  if (row_begin == 0) row_begin = 1; // row 0 has no cells
+  if (row_end <= row_begin) return;
+  std::FILE *f = std::fopen(filename.c_str(), append ? "ab" : "wb");
+  if (!f) throw std::runtime_error("Error: Failed to open file '" + filename + "'");
+  std::unique_ptr<std::FILE, int (*)(std::FILE *)> guard(f, std::fclose);
+  // Chunks of rows with about equal cell counts, formatted in parallel and
+  // written in row order.
+  const std::uint64_t c0 = row_begin * (row_begin - 1) / 2;
+  const std::uint64_t total = row_end * (row_end - 1) / 2 - c0;
+  const std::uint64_t chunks = std::max<std::uint64_t>(
+      1, std::min<std::uint64_t>(worker_count() * 4, total / (1u << 16) + 1));
+  std::vector<std::uint64_t> bounds{row_begin};
+  for (std::uint64_t k = 1; k < chunks; ++k) {
+    // first row whose start cell passes k/chunks of the band
+    const double target = static_cast<double>(c0) + static_cast<double>(total) * k / chunks;
+    std::uint64_t r = static_cast<std::uint64_t>(0.5 + std::sqrt(2.0 * target + 0.25));
+    r = std::max(r, bounds.back());
+    r = std::min(r, row_end);
+    bounds.push_back(r);
+  }
+  bounds.push_back(row_end);
+  const std::size_t nb = bounds.size() - 1;
+  std::vector<std::string> text(nb);
+  parallel_for(nb, [&](std::uint64_t b, std::uint64_t e) {
+    for (std::uint64_t k = b; k < e; ++k) {
+      const std::uint64_t r0 = bounds[k], r1 = bounds[k + 1];
+      if (r1 <= r0) continue;
+      const std::uint64_t cells_k = r1 * (r1 - 1) / 2 - r0 * (r0 - 1) / 2;
+      text[k].resize(cells_k * (kMaxDigits + 1) + 1);
+      const std::size_t len =
+          format_rows(cells + (r0 * (r0 - 1) / 2 - c0), r0, r1, text[k].data());
+      text[k].resize(len);
+    }
+  });
+  for (const auto &t : text) write_all(f, t.data(), t.size());
+}
+
+template void write_rows<std::uint8_t>(const std::string &, const std::uint8_t *,
+                                       std::uint64_t, std::uint64_t, bool);
+template void write_rows<std::uint16_t>(const std::string &, const std::uint16_t *,
+                                        std::uint64_t, std::uint64_t, bool);
+
+// write_lower_triangular, include/hamming/hamming_utils.hh:106-110: nothing
+// (no file) for an empty vector; a cell count that is not triangular ends in
+// an incomplete row terminated by ',' as partial_write_lower_triangular does.
+template <typename T>
+void write_lower_triangular(const std::string &filename, const std::vector<T> &cells) {
+  const std::uint64_t size = cells.size();
+  if (size == 0) return;
+  std::uint64_t rows = 1; // full rows 1..rows-1
+  while ((rows + 1) * rows / 2 <= size) ++rows;
+  write_rows(filename, cells.data(), 1, rows, false);
+  const std::uint64_t used = rows * (rows - 1) / 2;
+  if (used < size) {
+    if (rows == 1) {
+      std::FILE *f = std::fopen(filename.c_str(), "wb");
+      if (f) std::fclose(f);
+    }
+    std::FILE *f = std::fopen(filename.c_str(), "ab");
+    if (!f) throw std::runtime_error("Error: Failed to open file '" + filename + "'");
+    std::string tail;
+    char buf[16];
+    for (std::uint64_t x = used; x < size; ++x) {
+      char *e = put_uint(buf, static_cast<std::uint32_t>(cells[x]));
+      tail.append(buf, static_cast<std::size_t>(e - buf));
+      tail.push_back(',');
+    }
+    write_all(f, tail.data(), tail.size());
+    std::fclose(f);
+  }
+}
+
+template void write_lower_triangular<std::uint8_t>(const std::string &,
+                                                   const std::vector<std::uint8_t> &);
+template void write_lower_triangular<std::uint16_t>(const std::string &,
+                                                    const std::vector<std::uint16_t> &);
+
+// DataSet::dump_sparse, include/hamming/hamming.hh:79-114: "i j d\n" for each
+// cell with d <= threshold, in row-major order.  The cells are selected and
+// compacted on the GPU (hd_sparse_emit); the host formats the triplets.
+template <typename T>
+void DataSet<T>::dump_sparse(const std::string &filename, int threshold) const {
+  std::FILE *f = std::fopen(filename.c_str(), "wb");
+  if (!f) throw std::runtime_error("Error: Failed to open file '" + filename + "'");
+  std::unique_ptr<std::FILE, int (*)(std::FILE *)> guard(f, std::fclose);
+  // Whole rows only, as the reference's loop over i < nsamples.
+  const std::uint64_t rows = nsamples;
+  const std::uint64_t cells = rows < 2 ? 0 : rows * (rows - 1) / 2;
+  if (cells == 0 || threshold < 0 || cells > result.size()) {
+    if (cells > result.size())
+      throw std::runtime_error("Error: distance data smaller than nsamples implies");
+    return;
+  }
+  std::uint64_t count = 0;
+  int st = hd_sparse_emit(result.data(), rows, sizeof(T), threshold, nullptr, 0, &count);
+  if (st != HD_OK && st != HD_ERR_CAPACITY) check(st);
+  if (count == 0) return;
+  std::vector<std::uint32_t> trip(3 * count);
+  check(hd_sparse_emit(result.data(), rows, sizeof(T), threshold, trip.data(), count, &count));
+  const std::uint64_t chunks = std::min<std::uint64_t>(worker_count() * 4, count);
+  std::vector<std::string> text(chunks);
+  parallel_for(chunks, [&](std::uint64_t b, std::uint64_t e) {
+    for (std::uint64_t k = b; k < e; ++k) {
+      const std::uint64_t t0 = count * k / chunks, t1 = count * (k + 1) / chunks;
+      std::string &s = text[k];
+      s.resize((t1 - t0) * 3 * (kMaxDigits + 1));
+      char *p = s.data();
+      for (std::uint64_t t = t0; t < t1; ++t) {
+        p = put_uint(p, trip[3 * t]);
+        *p++ = ' ';
+        p = put_uint(p, trip[3 * t + 1]);
+        *p++ = ' ';
+        p = put_uint(p, trip[3 * t + 2]);
+        *p++ = '\n';
+      }
+      s.resize(static_cast<std::size_t>(p - s.data()));
+    }
+  });
+  for (const auto &t : text) write_all(f, t.data(), t.size());
+}
+
+template void DataSet<std::uint8_t>::dump_sparse(const std::string &, int) const;
+template void DataSet<std::uint16_t>::dump_sparse(const std::string &, int) const;
+
+} // namespace hdhost

@@ -1,0 +1,141 @@
+// Device kernels of the Hamming engine (gfx950, wave64, integer VALU).
+//
+// Data layout in HBM
+//   packed  : the reference's dense nibble rows, row-major, bytes_per_seq each
+//             (src/hamming_impl.cc:244-267).  Used by the reference-distance
+//             scan and as the upload format.
+//   planes  : bit-sliced, in groups of 64 rows:
+//             planes[g][k][p][64] uint32, row i = 64 g + lane.  Word k of plane
+//             p of row i holds bit p of the plane code of sites 32k .. 32k+31.
+//             A wavefront reading word (k, p) of one group reads 256
+//             contiguous bytes; the same word of 16 consecutive columns is one
+//             64-byte scalar load; all P planes of one step of a group lie in
+//             P*256 contiguous bytes, so every per-step offset is an
+//             instruction immediate.
+//   triangle: cell (i, j), i > j, at i*(i-1)/2 + j (64-bit index).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hd {
+
+constexpr int kWave = 64;
+
+struct Table16 {
+  uint8_t v[16];
+};
+
+// ---------------------------------------------------------------------------
+// all-pairs tile kernel
+//
+// A workgroup of NW waves owns a TI x TJ tile of the lower triangle
+// (TI = 64 R rows, TJ = NW C columns).  Lane l of every wave holds rows
+// i0 + l + 64 q (q < R) in VGPRs; wave w holds columns j0 + w C + c (c < C)
+// in SGPRs (wave-uniform scalar loads), so each VALU op pairs a per-lane row
+// word with a broadcast column word and no LDS traffic is needed.  Per pair
+// and per 32 sites the work is P-1 AND/AND-OR ops, one NOR-with-AND and one
+// popcount-accumulate: 5 ops for the 4-plane dense code.
+// ---------------------------------------------------------------------------
+template <int P, int R, int C, int NW, typename OutT>
+__global__ __launch_bounds__(kWave *NW) void k_allpairs(
+    const uint32_t *__restrict__ planes, uint32_t n, uint32_t W, uint64_t tile_begin, uint32_t n_tiles, OutT *__restrict__ out,
+    uint64_t cell_offset, uint32_t max_dist, uint32_t check_every) {
+  constexpr uint32_t TI = kWave * R;
+  constexpr uint32_t TJ = NW * C;
+  constexpr uint32_t M = TI / TJ;
+  static_assert(TI % TJ == 0, "tile rows must be a multiple of tile columns");
+
+  // The dispatcher hands consecutive workgroups to the 8 XCDs in turn; map
+  // them so that each XCD walks a contiguous run of tiles (same row block,
+  // neighbouring column blocks) and shares the row data in its own L2.
+  const uint32_t g = blockIdx.x;
+  const uint32_t q8 = n_tiles >> 3, r8 = n_tiles & 7u, x = g & 7u;
+  const uint64_t t =
+      tile_begin + (uint64_t)x * q8 + (x < r8 ? x : r8) + (g >> 3);
+
+  // Tile t -> (row block b, column block cb); row block b has M (b+1) tiles.
+  uint64_t b = (uint64_t)((sqrt(8.0 * (double)t / (double)M + 1.0) - 1.0) * 0.5);
+  while ((uint64_t)M * (b + 1) * (b + 2) / 2 <= t) ++b;
+  while (b > 0 && (uint64_t)M * b * (b + 1) / 2 > t) --b;
+  const uint32_t cb = (uint32_t)(t - (uint64_t)M * b * (b + 1) / 2);
+
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint32_t i0 = (uint32_t)b * TI;
+  const uint32_t j0 = cb * TJ + wave * C;
+  const uint32_t i_end = min(i0 + TI, n);
+  if (j0 + 1 >= i_end) return; // no pair j < i inside this wave's block
+
+  // Pairs outside the triangle start saturated so that the early-exit vote
+  // only waits for real pairs; they are never stored.
+  uint32_t acc[R][C];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const uint32_t i = i0 + kWave * q + lane;
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+      acc[q][c] = (j0 + c < i && i < n) ? 0u : max_dist;
+  }
+
+  // Row group g starts at word g * W * P * 64; one step of it is P * 64 words.
+  const size_t group = (size_t)W * P * kWave;
+  const uint32_t *pa = planes + (size_t)(i0 / kWave) * group + lane;
+  const uint32_t *pb = planes + (size_t)(j0 / kWave) * group + (j0 % kWave);
+  uint32_t k = 0;
+  while (k < W) {
+    const uint32_t k_end = min(W, k + check_every);
+    for (; k < k_end; ++k) {
+      uint32_t a[R][P];
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int p = 0; p < P; ++p) a[q][p] = pa[q * group + p * kWave];
+      uint32_t bw[C][P];
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int c = 0; c < C; ++c) bw[c][p] = pb[p * kWave + c];
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          uint32_t s = a[q][0] & bw[c][0];
+#pragma unroll
+          for (int p = 1; p < P - 1; ++p) s |= a[q][p] & bw[c][p];
+          s = ~(s | (a[q][P - 1] & bw[c][P - 1]));
+          acc[q][c] += __builtin_popcount(s);
+        }
+      pa += P * kWave;
+      pb += P * kWave;
+    }
+    if (check_every < W) {
+      uint32_t lo = acc[0][0];
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int c = 0; c < C; ++c) lo = min(lo, acc[q][c]);
+      if (__all(lo >= max_dist)) break;
+    }
+  }
+
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const uint32_t i = i0 + kWave * q + lane;
+    if (i < n) {
+      OutT *row = out + ((uint64_t)i * (i - 1) / 2 + j0 - cell_offset);
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+        if (j0 + c < i) row[c] = (OutT)min(acc[q][c], max_dist);
+    }
+  }
+}
+
+// Number of tiles of the rows blocks [0, rb) for an M column blocks per row
+// block tiling.
+inline uint64_t tiles_before(uint64_t rb, uint64_t M) {
+  return M * rb * (rb + 1) / 2;
+}
+
+} // namespace hd

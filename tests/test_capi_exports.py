@@ -1,0 +1,68 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares
+(no compute calls: these run without a GPU)."""
+import os
+import re
+
+from hammingdist_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\**\s*(hd_[a-z_0-9]+)\s*\(", text,
+                             flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert {"hd_all_pairs", "hd_all_pairs_stream", "hd_reference_distances",
+            "hd_distance_pair", "hd_device_count", "hd_last_error"} <= names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = capi.lib()
+    missing = [n for n in sorted(declared_functions()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_ctypes_signatures_cover_header():
+    assert declared_functions() <= set(capi.SIGNATURES)
+
+
+def test_no_device_is_a_clean_error():
+    if capi.device_count() > 0:
+        return
+    import numpy as np
+    import pytest
+
+    packed = np.full((2, 8), 0xFF, np.uint8)
+    out = np.zeros(1, np.uint8)
+    with pytest.raises(capi.EngineError, match="no HIP device"):
+        capi.all_pairs(packed, 2, 8, capi.HD_SEM_DENSE, 1, 255, out)
+
+
+def test_invalid_arguments_are_rejected_before_device_use():
+    import numpy as np
+    import pytest
+
+    packed = np.full((2, 8), 0xFF, np.uint8)
+    out = np.zeros(1, np.uint8)
+    with pytest.raises(capi.EngineError, match="max_dist"):
+        capi.all_pairs(packed, 2, 8, capi.HD_SEM_DENSE, 1, 256, out)
+    with pytest.raises(capi.EngineError, match="out_bytes"):
+        capi.all_pairs(packed, 2, 8, capi.HD_SEM_DENSE, 3, 1, out)
+    # planes layout is pure host arithmetic
+    n_pad, W, P, total = capi.planes_layout(1000, 14952, capi.HD_SEM_DENSE, 0xFFFF)
+    assert (n_pad, W, P) == (1024, 935, 4) and total == 1024 * 935 * 4
+    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, (1 << 3) | (1 << 0))[2] == 6
+    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, (1 << 1) | (1 << 15))[2] == 4
+    assert capi.planes_layout(10, 8, capi.HD_SEM_CHAR_X, 1 << 3)[2] == 5

@@ -1,0 +1,173 @@
+"""Parity of the HIP engine against the oracle, through the C ABI.
+
+Every comparison is exact (integer cells).  Sizes follow the reference's
+CPU==GPU tests (src/hamming_t.cc:346-451: lengths 1..1092, 2..703 samples
+around the 127/128/255/256/257 boundaries, max_dist {0,1,2,3,89,497,9999999})
+plus sizes that cross this engine's own tile boundaries (64-row groups,
+128/256/512-row tiles, 32-site words).
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import random_strings
+from hammingdist_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+DT = {1: np.uint8, 2: np.uint16, 4: np.uint32}
+
+
+def engine_all_pairs(packed, semantics, out_bytes, max_dist):
+    n, bps = packed.shape
+    out = np.zeros(n * (n - 1) // 2, DT[out_bytes])
+    capi.all_pairs(np.ascontiguousarray(packed), n, bps, semantics, out_bytes, max_dist, out)
+    return out
+
+
+def test_device_present():
+    assert capi.device_count() >= 1
+    print("device arch:", capi.device_arch())
+
+
+@pytest.mark.parametrize("n,L", [(2, 1), (3, 17), (7, 88), (11, 381), (127, 1023), (128, 33),
+                                 (255, 64), (256, 95), (257, 1092), (513, 130), (703, 381),
+                                 (1100, 700)])
+def test_dense_all_pairs_matches_oracle(n, L, rng):
+    seqs = random_strings(rng, n, L, "ACGT-")
+    packed = O.pack(O.as_chars(seqs))
+    for out_bytes, max_dist in [(1, 255), (2, 65535), (4, 2**32 - 1), (1, 3), (2, 89),
+                                (1, 0), (4, 497)]:
+        want = O.all_pairs_dense(packed, out_bytes, max_dist)
+        got = engine_all_pairs(packed, capi.HD_SEM_DENSE, out_bytes, max_dist)
+        np.testing.assert_array_equal(got, want, err_msg=f"n={n} L={L} md={max_dist}")
+
+
+@pytest.mark.parametrize("alphabet", ["ACGT-NX", "ACGT-", "AAAAAAAACGT-N", "ACGTXn*-?"])
+@pytest.mark.parametrize("n,L", [(50, 25), (300, 513), (700, 97)])
+def test_sparse_semantics_match_oracle(alphabet, n, L, rng):
+    seqs = random_strings(rng, n, L, alphabet)
+    chars = O.as_chars(seqs)
+    for include_x in (False, True):
+        packed = O.pack(chars, include_x)
+        sem = capi.HD_SEM_SPARSE_X if include_x else capi.HD_SEM_SPARSE
+        for out_bytes, md in [(1, 255), (2, 65535), (1, 5)]:
+            want = O.all_pairs_sparse(chars, include_x, out_bytes, md)
+            got = engine_all_pairs(packed, sem, out_bytes, md)
+            np.testing.assert_array_equal(got, want,
+                                          err_msg=f"{alphabet} x={include_x} md={md}")
+
+
+@pytest.mark.parametrize("alphabet", ["ACGT-NX", "ACGTXn*-?", "ACGT-"])
+def test_char_x_all_pairs_match_distance(alphabet, rng):
+    n, L = 60, 150
+    seqs = random_strings(rng, n, L, alphabet)
+    packed = O.pack(O.as_chars(seqs), include_x=True)
+    got = engine_all_pairs(packed, capi.HD_SEM_CHAR_X, 4, 2**32 - 1)
+    I, J = O.tri_index(n)
+    want = np.array([O.distance_char(seqs[i], seqs[j], True) for i, j in zip(I, J)], np.uint32)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_a_versus_g_closed_form():
+    # src/hamming_impl_t.cc:28-47: n x 'A' against n x 'G' -> min(max_dist, n)
+    for L in [1, 2, 31, 32, 33, 63, 64, 65, 255, 256, 257, 1023, 1024, 1025, 8193, 65537,
+              131073, 1048577]:
+        packed = O.pack(O.as_chars(["A" * L, "G" * L]))
+        for md in [0, 1, 2, 11, 999, 9876544]:
+            got = engine_all_pairs(packed, capi.HD_SEM_DENSE, 4, md)
+            assert int(got[0]) == min(md, L), (L, md)
+
+
+def test_identical_rows_are_zero(rng):
+    # src/hamming_impl_t.cc:6-26 and the CUDA variant src/distance_cuda_t.cc:6-26
+    for L in [1, 7, 8, 9, 16, 17, 4095, 4097, 32769]:
+        s = random_strings(rng, 1, L, "ACGT-")[0]
+        packed = O.pack(O.as_chars([s, s, s]))
+        for sem in (capi.HD_SEM_DENSE, capi.HD_SEM_SPARSE):
+            assert engine_all_pairs(packed, sem, 4, 2**31 - 1).tolist() == [0, 0, 0]
+
+
+def test_distance_pair_matches_oracle(rng):
+    # src/distance_cuda_t.cc:50-72: pair kernel == distance_cpp on random rows
+    for L in [1, 5, 17, 64, 65, 1000, 4097]:
+        a, b = random_strings(rng, 2, L, "ACGT-")
+        pa, pb = O.pack(O.as_chars([a])), O.pack(O.as_chars([b]))
+        for md in [0, 1, 2, 11, 999, 9876544]:
+            got = capi.distance_pair(pa[0], pb[0], pa.shape[1], capi.HD_SEM_DENSE, md)
+            assert got == O.distance_dense(pa[0], pb[0], md)
+
+
+def test_stream_bands_concatenate_to_triangle(rng):
+    n, L = 1500, 200
+    seqs = random_strings(rng, n, L, "ACGT-N")
+    packed = np.ascontiguousarray(O.pack(O.as_chars(seqs)))
+    want = O.all_pairs_dense(packed, 2, 65535)
+    got = np.zeros_like(want)
+    seen = []
+
+    def cb(addr, first, count, r0, r1):
+        import ctypes
+        buf = (ctypes.c_uint16 * count).from_address(addr)
+        got[first:first + count] = np.frombuffer(buf, np.uint16)
+        assert first == r0 * (r0 - 1) // 2
+        assert first + count == r1 * (r1 - 1) // 2
+        seen.append((r0, r1))
+
+    capi.all_pairs_stream(packed, n, packed.shape[1], capi.HD_SEM_DENSE, 2, 65535, 100_000, cb)
+    assert len(seen) > 3
+    assert seen[0][0] == 1 and seen[-1][1] == n
+    np.testing.assert_array_equal(got, want)
+
+
+def test_reference_distances_match_oracle(rng):
+    m, L = 400, 1234
+    for alphabet, include_x in [("ACGT-", False), ("ACGT-NX", False), ("ACGT-NX", True),
+                                ("ACGTXn*-?", True)]:
+        seqs = random_strings(rng, m + 1, L, alphabet)
+        ref, rows = seqs[0], seqs[1:]
+        db = np.ascontiguousarray(O.pack(O.as_chars(rows), include_x))
+        q = np.ascontiguousarray(O.pack(O.as_chars([ref]), include_x)[0])
+        sem = capi.HD_SEM_CHAR_X if include_x else capi.HD_SEM_DENSE
+        out = np.zeros(m, np.uint32)
+        capi.reference_distances(q, db, m, db.shape[1], None, m, sem, out)
+        want = [O.distance_char(ref, s, include_x) for s in rows]
+        assert out.tolist() == want
+        # gathered subset, with repeats
+        sel = rng.integers(0, m, size=257).astype(np.uint64)
+        out2 = np.zeros(sel.size, np.uint32)
+        capi.reference_distances(q, db, m, db.shape[1], sel, sel.size, sem, out2)
+        assert out2.tolist() == [want[int(k)] for k in sel]
+
+
+def test_consensus_diff_matches_oracle(rng):
+    for alphabet in ["ACGT-", "AAAAAAAAAAAAAAAACGTN-", "ACGTX-"]:
+        n, L = 333, 1001
+        seqs = random_strings(rng, n, L, alphabet)
+        chars = O.as_chars(seqs)
+        packed = np.ascontiguousarray(O.pack(chars))
+        n_diff, mask = capi.consensus_diff(packed, n, packed.shape[1], L)
+        cons = np.frombuffer(O.consensus(chars, False), np.uint8)
+        assert n_diff == int((chars != cons[None, :]).sum())
+        codes = set()
+        for ch in set(alphabet):
+            v = {"A": 1, "C": 2, "G": 4, "T": 8, "-": 15}.get(ch, 0)
+            codes.add(v)
+        assert mask == sum(1 << c for c in codes)
+
+
+@pytest.mark.parametrize("out_bytes", [1, 2, 4])
+def test_sparse_emit_row_major(out_bytes, rng):
+    n = 900
+    cells = rng.integers(0, 40, size=n * (n - 1) // 2).astype(DT[out_bytes])
+    I, J = O.tri_index(n)
+    for thr in [-1, 0, 3, 39, 1000]:
+        sel = cells <= thr
+        count = capi.sparse_emit(cells, n, out_bytes, thr, None, 0)
+        assert count == int(sel.sum())
+        if count:
+            trip = np.zeros((count, 3), np.uint32)
+            capi.sparse_emit(cells, n, out_bytes, thr, trip, count)
+            np.testing.assert_array_equal(trip[:, 0], I[sel])
+            np.testing.assert_array_equal(trip[:, 1], J[sel])
+            np.testing.assert_array_equal(trip[:, 2], cells[sel])
