@@ -53,23 +53,32 @@ def cpu_baseline(packed, L: int, target_s: float = 12.0):
     def run(r0, r1, md, out_bytes=2):
         cells = r1 * (r1 - 1) // 2 - r0 * (r0 - 1) // 2
         out = np.zeros(cells, np.uint16 if out_bytes == 2 else np.uint8)
-        t = time.perf_counter()
         if ref is not None:
+            secs = C.c_double(0.0)
             used = ref.ref_all_pairs(3, packed.ctypes.data, n, bps, r0, r1, out_bytes, md,
-                                     out.ctypes.data, threads)
-        else:
-            used = O.lib().oracle_all_pairs_dense_simd(packed.ctypes.data, n, bps, r0, r1,
-                                                       out_bytes, md, out.ctypes.data, threads)
+                                     out.ctypes.data, threads, C.byref(secs))
+            return secs.value, cells, used, out
+        t = time.perf_counter()
+        used = O.lib().oracle_all_pairs_dense_simd(packed.ctypes.data, n, bps, r0, r1,
+                                                   out_bytes, md, out.ctypes.data, threads)
         return time.perf_counter() - t, cells, used, out
 
-    # calibrate on a small band, then size the sample to about target_s
+    def rows_for(r0, want_cells):
+        r1 = r0
+        while r1 < n and (r1 * (r1 - 1) // 2 - r0 * (r0 - 1) // 2) < want_cells:
+            r1 += 8
+        return r1
+
+    # calibrate on a band of about 1 s, then size the sample to about target_s
     r0 = 1000
-    dt, cells, used, _ = run(r0, r0 + 40, 65535)
+    dt, cells, used, _ = run(r0, r0 + 64, 65535)
+    for _ in range(3):
+        if dt >= 0.5:
+            break
+        r1 = rows_for(r0, cells * min(64.0, 1.0 / max(dt, 1e-4)))
+        dt, cells, used, _ = run(r0, r1, 65535)
     rate = cells / max(dt, 1e-6)
-    want_cells = rate * target_s
-    r1 = r0
-    while r1 < n and (r1 * (r1 - 1) // 2 - r0 * (r0 - 1) // 2) < want_cells:
-        r1 += 16
+    r1 = rows_for(r0, rate * target_s)
     dt, cells, used, out = run(r0, r1, 65535)
     site_rate = cells * L / dt
     dt2, cells2, _, _ = run(r0, min(n, r0 + (r1 - r0) // 4 + 16), 2, 1)

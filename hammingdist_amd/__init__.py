@@ -15,6 +15,10 @@ import os as _os
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
 
+from ._runtime import ensure_single_runtime as _ensure_single_runtime
+
+_ensure_single_runtime()
+
 try:
     from . import _hammingdist as _native
 except ImportError as exc:  # pragma: no cover - exercised only when unbuilt

@@ -74,6 +74,9 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIBRARY):
             raise ImportError(f"{LIBRARY} is not built; run hammingdist_amd/build.py")
+        from ._runtime import ensure_single_runtime
+
+        ensure_single_runtime()
         handle = C.CDLL(LIBRARY)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

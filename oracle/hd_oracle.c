@@ -29,6 +29,9 @@
 
 #define GAP 0xFFu
 
+/* A uint32 cell bound as the int max_dist of the reference kernels. */
+static int md_int(uint32_t m) { return m > 0x7FFFFFFFu ? 0x7FFFFFFF : (int)m; }
+
 /* lookupTable, src/hamming_impl.cc:31-44 */
 void oracle_lookup(int include_x, uint8_t out[256]) {
   memset(out, 0, 256);
@@ -209,7 +212,7 @@ void oracle_all_pairs_sparse(const char *data, uint64_t n, uint64_t L, int inclu
     for (int64_t j = 0; j < i; ++j)
       store_cell(out, out_bytes, off + (uint64_t)j,
                  (uint32_t)sparse_merge(idx[i], code[i], len[i], idx[j], code[j],
-                                        len[j], (int)max_dist));
+                                        len[j], md_int(max_dist)));
   }
   for (uint64_t s = 0; s < n; ++s) {
     free(idx[s]);
@@ -232,7 +235,7 @@ void oracle_all_pairs_dense(const uint8_t *packed, uint64_t n, uint64_t bytes_pe
       store_cell(out, out_bytes, off + (uint64_t)j,
                  (uint32_t)oracle_distance_dense(packed + (uint64_t)i * bytes_per_seq,
                                                  packed + (uint64_t)j * bytes_per_seq,
-                                                 bytes_per_seq, (int)max_dist));
+                                                 bytes_per_seq, md_int(max_dist)));
   }
 }
 
@@ -330,10 +333,10 @@ int oracle_all_pairs_dense_simd(const uint8_t *packed, uint64_t n, uint64_t byte
       const uint8_t *b = packed + (uint64_t)j * bytes_per_seq;
       int d;
 #if defined(__x86_64__)
-      if (simd) d = dist_avx512(a, b, bytes_per_seq, (int)max_dist);
-      else d = oracle_distance_dense(a, b, bytes_per_seq, (int)max_dist);
+      if (simd) d = dist_avx512(a, b, bytes_per_seq, md_int(max_dist));
+      else d = oracle_distance_dense(a, b, bytes_per_seq, md_int(max_dist));
 #else
-      d = oracle_distance_dense(a, b, bytes_per_seq, (int)max_dist);
+      d = oracle_distance_dense(a, b, bytes_per_seq, md_int(max_dist));
 #endif
       store_cell(out, out_bytes, off + (uint64_t)j, (uint32_t)d);
     }

@@ -11,6 +11,7 @@
 // not built; the all-pairs loop below restates distances<T>'s dense CPU route
 // (include/hamming/hamming_impl.hh:141-173: OpenMP schedule(static,1) over
 // rows, cell i*(i-1)/2 + j).
+#include <chrono>
 #include <cstdint>
 #include <vector>
 
@@ -53,14 +54,17 @@ int ref_distance(int which, const uint8_t *a, const uint8_t *b, uint64_t bytes,
 }
 
 // Rows [row0, row1) of the dense all-pairs triangle with the reference kernel
-// `which`; cells are written relative to row0's first cell.  Returns the
-// OpenMP thread count used.
+// `which`; cells are written relative to row0's first cell.  The rows are
+// copied into the reference's vector-of-vectors form first (as to_dense_data
+// does before the timed distance loop); *seconds receives the loop's wall
+// time.  Returns the OpenMP thread count used.
 int ref_all_pairs(int which, const uint8_t *packed, uint64_t n, uint64_t bps,
                   uint64_t row0, uint64_t row1, int out_bytes, int max_dist,
-                  void *out, int threads) {
+                  void *out, int threads, double *seconds) {
   if (threads > 0) omp_set_num_threads(threads);
   const int used = omp_get_max_threads();
-  const std::vector<Row> rows = rows_of(packed, n, bps);
+  const std::vector<Row> rows = rows_of(packed, row1 < n ? row1 : n, bps);
+  const auto t0 = std::chrono::steady_clock::now();
   const Kernel f = pick(which);
   const uint64_t base = row0 ? row0 * (row0 - 1) / 2 : 0;
 #pragma omp parallel for schedule(static, 1)
@@ -73,6 +77,8 @@ int ref_all_pairs(int which, const uint8_t *packed, uint64_t n, uint64_t bps,
       else static_cast<uint32_t *>(out)[off + j] = (uint32_t)d;
     }
   }
+  if (seconds)
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return used;
 }
 }

@@ -57,7 +57,7 @@ def ref():
         _ref.ref_distance.argtypes = [C.c_int, vp, vp, u64, C.c_int]
         _ref.ref_all_pairs.restype = C.c_int
         _ref.ref_all_pairs.argtypes = [C.c_int, vp, u64, u64, u64, u64, C.c_int, C.c_int, vp,
-                                       C.c_int]
+                                       C.c_int, C.POINTER(C.c_double)]
     return _ref
 
 

@@ -1,0 +1,168 @@
+"""Host-side logic of the package that runs without a GPU: FASTA ingest and
+dedup, loaders, text writers, error messages and argument checks.  The
+oracle (oracle/fasta_oracle.py, oracle/hd_oracle.c writers) is the checker."""
+import numpy as np
+import pytest
+
+import hammingdist_amd as hd
+import oracle_lib as O
+from conftest import random_strings
+from oracle.fasta_oracle import fasta_indices, read_fasta
+
+
+def write_fasta(path, seqs):
+    with open(path, "w") as f:
+        for i, s in enumerate(seqs):
+            f.write(f">seq{i}\n{s}\n")
+
+
+@pytest.mark.parametrize("content", [
+    ">a\nAC\n>b\nAC\n>c\nGT\n",
+    ">a\nAC\n>b\n",
+    ">a\nAC\n>b",
+    ">a\nAC\n\nGT\n>b\nACGT\n>c\nTT\n",
+    "",
+    "no header\nAC\n>x\nAC",
+    ">a\r\nAC\r\n>b\r\nAC\r\n",
+    ">only\n",
+])
+@pytest.mark.parametrize("n", [0, 1, 2, 5])
+def test_sequence_indices_match_oracle(tmp_path, content, n):
+    p = str(tmp_path / "f.fa")
+    open(p, "w").write(content)
+    got = hd.fasta_sequence_indices(p, n).tolist()
+    assert got == fasta_indices(p, n)
+    assert hd.fasta_sequence_indices(p, n).dtype == np.uint64
+
+
+def test_sequence_indices_random_dedup(tmp_path, rng):
+    base = random_strings(rng, 30, 40, "ACGT-N")
+    seqs = [base[k] for k in rng.integers(0, 30, size=200)]
+    p = str(tmp_path / "f.fa")
+    write_fasta(p, seqs)
+    assert hd.fasta_sequence_indices(p).tolist() == fasta_indices(p)
+    assert hd.fasta_sequence_indices(p, 77).tolist() == fasta_indices(p, 77)
+
+
+def test_missing_file_is_an_error(tmp_path):
+    missing = str(tmp_path / "nope.fa")
+    with pytest.raises(RuntimeError, match="Failed to open file"):
+        hd.fasta_sequence_indices(missing)
+    with pytest.raises(RuntimeError, match="Failed to open file"):
+        hd.fasta_reference_distances("ACGT", missing)
+    with pytest.raises(RuntimeError, match="Failed to open file"):
+        hd.from_fasta(missing)
+
+
+def test_reference_error_messages():
+    for data in ([], ["", ""]):
+        with pytest.raises(RuntimeError, match="^Error: Empty sequence$"):
+            hd.from_stringlist(data, False, False, 255)
+    for data in (["ACGT", "A"], ["AC", "ACGTCG"], ["ACGT"] * 5 + ["A"], ["ACGT", "A-----"],
+                 ["ACGT", ""]):
+        with pytest.raises(RuntimeError,
+                           match="^Error: Sequences do not all have the same length$"):
+            hd.from_stringlist(data, False, False, 255)
+        with pytest.raises(RuntimeError,
+                           match="^Error: Sequences do not all have the same length$"):
+            hd.distance(data[0], data[-1])
+    with pytest.raises(RuntimeError, match="use_gpu=True cannot be used if include_x=True"):
+        hd.from_stringlist(["ACGT", "ACGA"], True, True, 255)
+
+
+def test_stream_requires_gpu_flag(tmp_path):
+    p = str(tmp_path / "f.fa")
+    write_fasta(p, ["ACGT", "ACGA"])
+    with pytest.raises(RuntimeError,
+                       match="from_fasta_to_lower_triangular is currently only available on GPU"):
+        hd.from_fasta_to_lower_triangular(p, str(tmp_path / "o.txt"), use_gpu=False)
+
+
+def test_positional_signatures():
+    # from_stringlist has no defaults (python/hammingdist.cc:69-70)
+    with pytest.raises(TypeError):
+        hd.from_stringlist(["AC", "AG"])
+
+
+def lt_file(path, rows):
+    with open(path, "w") as f:
+        for r in rows:
+            f.write(",".join(str(v) for v in r) + "\n")
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 11, 54, 120])
+def test_lower_triangular_roundtrip_and_writer(tmp_path, n, rng):
+    rows = [rng.integers(0, 300, size=i).tolist() for i in range(1, n)]
+    src = str(tmp_path / "lt.txt")
+    lt_file(src, rows)
+    for loader, dt, cap in [(hd.from_lower_triangular, np.uint8, 255),
+                            (hd.from_lower_triangular_large, np.uint16, 65535)]:
+        ds = loader(src)
+        cells = np.array([min(v, cap) for r in rows for v in r], dt)
+        assert ds.lt_array.dtype == dt
+        np.testing.assert_array_equal(ds.lt_array, cells)
+        assert ds._distances == cells.tolist()
+        out = str(tmp_path / "out.txt")
+        ds.dump_lower_triangular(out)
+        ref_out = str(tmp_path / "ref.txt")
+        O.lib().oracle_write_lower_triangular(ref_out.encode(), cells.ctypes.data, n,
+                                              cells.itemsize)
+        assert open(out).read() == open(ref_out).read()
+        full = str(tmp_path / "full.txt")
+        ds.dump(full)
+        O.lib().oracle_write_full(ref_out.encode(), cells.ctypes.data, n, cells.itemsize)
+        assert open(full).read() == open(ref_out).read()
+        for i in range(n):
+            for j in range(n):
+                a, b = max(i, j), min(i, j)
+                want = 0 if i == j else int(cells[a * (a - 1) // 2 + b])
+                assert ds[i, j] == want
+        with pytest.raises(IndexError):
+            ds[n, 0]
+
+
+def test_dump_sequence_indices_without_dedup(tmp_path):
+    src = str(tmp_path / "lt.txt")
+    lt_file(src, [[1], [2, 3]])
+    ds = hd.from_lower_triangular(src)
+    out = str(tmp_path / "idx.txt")
+    ds.dump_sequence_indices(out)
+    assert open(out).read() == "0\n1\n2\n"
+
+
+def test_from_csv_quirk_and_roundtrip(tmp_path, rng):
+    # DataSet(filename) sizes the result n(n+1)/2 (include/hamming/hamming.hh:40)
+    n = 9
+    rows = [rng.integers(0, 200, size=i).tolist() for i in range(1, n)]
+    src = str(tmp_path / "lt.txt")
+    lt_file(src, rows)
+    ds = hd.from_lower_triangular(src)
+    csv = str(tmp_path / "m.csv")
+    ds.dump(csv)
+    back = hd.from_csv(csv)
+    assert back.lt_array.shape == (n * (n + 1) // 2,)
+    for i in range(n):
+        for j in range(n):
+            assert back[i, j] == ds[i, j]
+
+
+def test_empty_lower_triangle_writes_no_file(tmp_path):
+    src = str(tmp_path / "lt.txt")
+    open(src, "w").write("")
+    ds = hd.from_lower_triangular(src)
+    out = tmp_path / "out.txt"
+    ds.dump_lower_triangular(str(out))
+    assert not out.exists()
+
+
+def test_generator_is_deterministic():
+    from hammingdist_amd import workloads
+
+    s = workloads.plain(n=50, L=300, seed=77)
+    a, b = workloads.generate_chars(s), workloads.generate_chars(s)
+    np.testing.assert_array_equal(a, b)
+    p = workloads.generate_packed(s)
+    np.testing.assert_array_equal(p, O.pack(a))
+    txt = {bytes(r) for r in a}
+    assert len(txt) < 50  # duplicates present
+    assert set(np.unique(a).tobytes()) <= set(b"ACGTN-")
