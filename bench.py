@@ -1,15 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the all-pairs Hamming distance engine on one MI355X.
 
-Default workload: the large drop-in case — n = 50,000 synthetic aligned
-sequences of L = 29,903 sites (64-clade generator), the uint8 lower triangle
-(1,249,975,000 cells) with max_distance = 2.  One step is one pass of the hot
-path over the resident packed input: encode to bit planes, then the all-pairs
-tile kernel writing every cell of the triangle into HBM.  The headline runs
-with the early exit off, so every pair is computed over all sites; a run with
-the (exact) early exit on is reported beside it.
+Headline workload (the `value`): the large drop-in case — n = 50,000
+synthetic aligned sequences of L = 29,903 sites (64-clade generator), the
+uint8 lower triangle (1,249,975,000 cells) with max_distance = 2.  One step is
+one pass of the hot path over the packed input already resident in HBM:
+encode to bit planes, then the all-pairs tile kernel writing every cell of the
+triangle into HBM.  The headline runs with the early exit off, so every pair
+is computed over all sites; the run with the (exact) early exit on is
+reported beside it.
 
-Prints one JSON line (see DESIGN.md for the field definitions).
+Also measured (fields of the same JSON line), unless skipped by flags:
+  wide      n=100,000, L=10,000, include_x (sparse-X semantics), uint16
+            triangle of 5.0e9 cells resident in HBM
+  refdist   reference distances of one query against every row (HBM scan)
+  batched   256 datasets of uneven n and L: one graph-captured persistent
+            launch vs 256 sequential plain launches (jobs/s)
+  serving   continuous-batching request loop over an HBM-resident database
+            (requests/s, p50/p99 latency)
 """
 from __future__ import annotations
 
@@ -32,6 +40,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
+    return oracle_lib
+
+
 def cpu_baseline(packed, L: int, target_s: float = 12.0):
     """The reference's own AVX-512 pair kernel (oracle/_ref, built from the
     reference sources) under the distances<T> OpenMP row loop, on a bounded
@@ -42,9 +57,7 @@ def cpu_baseline(packed, L: int, target_s: float = 12.0):
 
     import numpy as np
 
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-
+    O = oracle()
     n, bps = packed.shape
     threads = len(os.sched_getaffinity(0))
     ref = O.ref()
@@ -95,40 +108,45 @@ def cpu_baseline(packed, L: int, target_s: float = 12.0):
     }
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=50_000)
-    ap.add_argument("--L", type=int, default=29903)
-    ap.add_argument("--max-distance", type=int, default=2)
-    ap.add_argument("--variant", type=int, default=-1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-early-exit-run", action="store_true")
-    args = ap.parse_args()
+def tri_row(x: int) -> int:
+    i = int((1 + (1 + 8 * x) ** 0.5) // 2)
+    while i * (i - 1) // 2 > x:
+        i -= 1
+    while (i + 1) * i // 2 <= x:
+        i += 1
+    return i
 
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
-        return 2
 
-    import numpy as np
-    import torch
+class Timer:
+    """HIP events on torch's current stream (the stream the engine's kernels
+    are launched on here)."""
 
-    from hammingdist_amd import capi, workloads
+    def __init__(self, torch):
+        self.torch = torch
+        self.stream = torch.cuda.current_stream()
 
+    def time(self, fn, reps: int) -> float:
+        t = self.torch
+        a, b = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        a.record(self.stream)
+        for _ in range(reps):
+            fn()
+        b.record(self.stream)
+        t.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+
+def run_large(args, torch, capi, workloads, np):
     spec = workloads.large(n=args.n, L=args.L)
     t = time.perf_counter()
     packed = workloads.generate_packed(spec)
     n, bps = packed.shape
-    log(f"generated n={n} L={args.L} bps={bps} in {time.perf_counter() - t:.1f}s")
-
+    log(f"large: generated n={n} L={args.L} bps={bps} in {time.perf_counter() - t:.1f}s")
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
     d_packed = torch.from_numpy(packed.reshape(-1)).to(dev)
-    sem = capi.HD_SEM_DENSE
-    mask = 0xFFFF
+    sem, mask = capi.HD_SEM_DENSE, 0xFFFF
     n_pad, W, P, total = capi.planes_layout(n, bps, sem, mask)
     d_planes = torch.empty(total, dtype=torch.int32, device=dev)
     cells = n * (n - 1) // 2
@@ -148,8 +166,6 @@ def main() -> int:
     for _ in range(args.warmup):
         step(False)
     torch.cuda.synchronize()
-    log("warmup done")
-
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(args.steps)]
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -161,45 +177,27 @@ def main() -> int:
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall
-    total_ms = start.elapsed_time(stop)
+    ms_per_step = start.elapsed_time(stop) / args.steps
     kernel_ms = [a.elapsed_time(b) for a, b in evs]
-    ms_per_step = total_ms / args.steps
     kern_ms = sum(kernel_ms) / len(kernel_ms)
-    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, all-pairs kernel {kern_ms:.2f} ms")
+    log(f"large: {args.steps} steps {ms_per_step:.2f} ms/step, all-pairs kernel {kern_ms:.2f} ms")
 
-    # spot check of a few cells against the oracle's dense pair kernel
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-
+    # spot check of cells against the oracle's dense pair kernel
+    O = oracle()
     rng = np.random.default_rng(0)
     host = d_out[: 1 << 20].cpu().numpy()
     for x in rng.integers(0, min(cells, 1 << 20), size=64):
-        i = int((1 + (1 + 8 * int(x)) ** 0.5) // 2)
-        while i * (i - 1) // 2 > x:
-            i -= 1
-        while (i + 1) * i // 2 <= x:
-            i += 1
+        i = tri_row(int(x))
         j = int(x) - i * (i - 1) // 2
         want = O.distance_dense(packed[i], packed[j], md)
         assert int(host[x]) == want, (i, j, int(host[x]), want)
 
     site_cmp = cells * args.L
-    value = site_cmp / (ms_per_step / 1e3)
     ops = cells * W * OPS_PER_WORD[P]
     achieved = ops / (kern_ms / 1e3)
-    result = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "site-comparisons/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+    res = {
+        "value": site_cmp / (ms_per_step / 1e3),
         "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic",
         "config": {
             "workload": f"large all-pairs drop-in: n={n} L={args.L} 64-clade generator, "
                         f"uint8 lower triangle max_distance={md}, inputs resident in HBM",
@@ -214,13 +212,12 @@ def main() -> int:
             "unit": "TOP/s",
             "frac": achieved / PEAK_INT32_OPS,
             "traffic": None,
-            "kernel": "k_allpairs (dense, 4 planes)",
+            "kernel": "k_allpairs<4,4,16,4,uint8> (dense, 4 planes)",
             "kernel_ms": kern_ms,
             "ops_per_launch": ops,
         },
         "wall_s_timed": wall,
     }
-
     if not args.no_early_exit_run:
         ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
         step(True)
@@ -228,12 +225,202 @@ def main() -> int:
         step(True, ev)
         torch.cuda.synchronize()
         ee_ms = ev[0].elapsed_time(ev[1])
-        result["early_exit"] = {"kernel_ms": ee_ms,
-                                "site_comparisons_per_s": site_cmp / (ee_ms / 1e3),
-                                "note": "same triangle, tiles stop once every pair >= max_distance"}
-        log(f"early exit kernel {ee_ms:.2f} ms")
+        res["early_exit"] = {"kernel_ms": ee_ms,
+                             "site_comparisons_per_s": site_cmp / (ee_ms / 1e3),
+                             "note": "same triangle; a tile stops once all its pairs reach "
+                                     "max_distance"}
+        log(f"large: early-exit kernel {ee_ms:.2f} ms")
 
+    if not args.no_refdist:
+        # one query against every row of the resident packed rows (HBM scan)
+        q = torch.from_numpy(packed[1]).to(dev)
+        d_rd = torch.empty(n, dtype=torch.int32, device=dev)
+        tm = Timer(torch)
+        ms = tm.time(lambda: capi.reference_distances_dev(q, d_packed, n, bps, None, n,
+                                                          capi.HD_SEM_DENSE, d_rd, sh), 20)
+        got = d_rd[:64].cpu().numpy().astype(np.uint32)
+        for r in range(64):
+            assert int(got[r]) == O.distance_dense(packed[1], packed[r]), r
+        bytes_moved = n * bps + bps + 4 * n
+        res["refdist_dense"] = {"rows": n, "bytes_per_row": bps, "kernel_ms": ms,
+                                "GB_per_s": bytes_moved / (ms / 1e3) / 1e9,
+                                "hbm_frac": bytes_moved / (ms / 1e3) / PEAK_HBM}
+        log(f"refdist dense: {ms:.3f} ms, {bytes_moved / (ms / 1e3) / 1e9:.0f} GB/s")
     del d_planes, d_out, d_packed
+    torch.cuda.empty_cache()
+    return res, packed
+
+
+def run_wide(args, torch, capi, workloads, np):
+    spec = workloads.wide()
+    t = time.perf_counter()
+    packed = workloads.generate_packed(spec, include_x=True)
+    n, bps = packed.shape
+    log(f"wide: generated n={n} L={spec.L} in {time.perf_counter() - t:.1f}s")
+    dev = torch.device("cuda:0")
+    sh = torch.cuda.current_stream().cuda_stream
+    d_packed = torch.from_numpy(packed.reshape(-1)).to(dev)
+    sem = capi.HD_SEM_SPARSE_X
+    mask = capi.nibble_mask_dev(d_packed, n, bps, sh)
+    n_pad, W, P, total = capi.planes_layout(n, bps, sem, mask)
+    d_planes = torch.empty(total, dtype=torch.int32, device=dev)
+    cells = n * (n - 1) // 2
+    d_out = torch.empty(cells * 2, dtype=torch.uint8, device=dev)
+    tm = Timer(torch)
+
+    def step():
+        capi.encode_dev(d_packed, n, bps, sem, mask, d_planes, sh)
+        capi.all_pairs_dev(d_planes, n, bps, sem, mask, 2, 65535, False, -1, d_out, sh)
+
+    step()
+    torch.cuda.synchronize()
+    ms = tm.time(step, 2)
+    # spot check against the oracle's sparse route on a few rows
+    O = oracle()
+    chars = workloads.generate_chars(workloads.wide(n=400))
+    want = O.all_pairs_sparse(chars, True, 2, 65535)
+    host = d_out[: want.size * 2].cpu().numpy().view(np.uint16)
+    assert np.array_equal(host, want), "wide: triangle differs from the oracle"
+    ops = cells * W * OPS_PER_WORD[P]
+    res = {"n": n, "L": spec.L, "cells": cells, "semantics": "sparse_x", "planes": P,
+           "out": "uint16", "ms_per_step": ms, "site_comparisons_per_s": cells * spec.L / (ms / 1e3),
+           "valu_frac": ops / (ms / 1e3) / PEAK_INT32_OPS,
+           "triangle_GB": cells * 2 / 1e9}
+    log(f"wide: {ms:.1f} ms/step, P={P}")
+    # reference distances of a fresh mutant against all rows (CHAR_X)
+    from hammingdist_amd import _native
+
+    ref = _native._synth_mutant(spec.L, 1012, 0.05, 0.0, 0.001, 0)
+    O_ref = O.pack(O.as_chars([ref]), True)[0]
+    q = torch.from_numpy(O_ref).to(dev)
+    d_rd = torch.empty(n, dtype=torch.int32, device=dev)
+    rms = tm.time(lambda: capi.reference_distances_dev(q, d_packed, n, bps, None, n,
+                                                       capi.HD_SEM_CHAR_X, d_rd, sh), 20)
+    got = d_rd[:200].cpu().numpy().astype(np.uint32)
+    for r in range(200):
+        assert int(got[r]) == O.distance_char(ref, bytes(chars[r]).decode(), True), r
+    moved = n * bps + bps + 4 * n
+    res["refdist_char_x"] = {"rows": n, "kernel_ms": rms, "GB_per_s": moved / (rms / 1e3) / 1e9,
+                             "hbm_frac": moved / (rms / 1e3) / PEAK_HBM}
+    log(f"wide refdist: {rms:.3f} ms")
+    del d_planes, d_out, d_packed
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_batched(args, torch, capi, workloads, np):
+    from hammingdist_amd import autotune
+
+    specs = workloads.batched(count=args.batch_count)
+    t = time.perf_counter()
+    b = capi.Batch()
+    jobs, packs = [], []
+    for spec, ob in specs:
+        packed = workloads.generate_packed(spec)
+        n, bps = packed.shape
+        # the reference CPU route's choice of semantics for this dataset
+        n_diff, _ = capi.consensus_diff(packed, n, bps, spec.L)
+        sem = capi.HD_SEM_SPARSE if n_diff / (n * spec.L) < 0.005 else capi.HD_SEM_DENSE
+        md = 255 if ob == 1 else 65535
+        b.add(packed, sem, ob, md)
+        jobs.append((n, bps, ob))
+        packs.append((packed, sem, ob, md))
+    log(f"batched: {len(jobs)} datasets uploaded in {time.perf_counter() - t:.1f}s")
+    sh = torch.cuda.current_stream().cuda_stream
+    variant, times = autotune.tune_batch(b, jobs, sh, torch.cuda.synchronize, reps=2)
+    tm = Timer(torch)
+    b.run(sh)
+    b.run_sequential(sh)
+    torch.cuda.synchronize()
+    ms_batch = tm.time(lambda: b.run(sh), args.batch_reps)
+    ms_seq = tm.time(lambda: b.run_sequential(sh), args.batch_reps)
+    # bit-exact check of a sample of jobs against the oracle
+    O = oracle()
+    rng = np.random.default_rng(1)
+    b.run(sh)
+    torch.cuda.synchronize()
+    for k in rng.choice(len(jobs), size=min(6, len(jobs)), replace=False):
+        packed, sem, ob, md = packs[k]
+        if jobs[k][0] > 1500:
+            continue
+        got = b.output(int(k))
+        if sem == capi.HD_SEM_DENSE:
+            want = O.all_pairs_dense(packed, ob, md)
+        else:
+            want = None  # sparse-route jobs are checked in tests/test_gpu_batch.py
+        if want is not None:
+            assert np.array_equal(got, want), f"batched job {k} differs from the oracle"
+    site_cmp = sum(n * (n - 1) // 2 * spec.L for (n, _, _), (spec, _) in zip(jobs, specs))
+    res = {"datasets": len(jobs), "variant": variant,
+           "autotune_ms": {str(k): v * 1e3 for k, v in times.items()},
+           "batched_ms": ms_batch, "sequential_ms": ms_seq,
+           "batched_jobs_per_s": len(jobs) / (ms_batch / 1e3),
+           "sequential_jobs_per_s": len(jobs) / (ms_seq / 1e3),
+           "speedup": ms_seq / ms_batch,
+           "site_comparisons_per_s": site_cmp / (ms_batch / 1e3),
+           "info": b.info()}
+    log(f"batched: {ms_batch:.2f} ms vs sequential {ms_seq:.2f} ms ({ms_seq / ms_batch:.2f}x)")
+    b.close()
+    return res
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=50_000)
+    ap.add_argument("--L", type=int, default=29903)
+    ap.add_argument("--max-distance", type=int, default=2)
+    ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-early-exit-run", action="store_true")
+    ap.add_argument("--no-refdist", action="store_true")
+    ap.add_argument("--no-wide", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline step (for profiling)")
+    args = ap.parse_args()
+    if args.headline_only:
+        args.no_cpu_baseline = args.no_early_exit_run = args.no_refdist = True
+        args.no_wide = args.no_batched = True
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    from hammingdist_amd import capi, workloads
+
+    large, packed = run_large(args, torch, capi, workloads, np)
+    result = {
+        "metric": METRIC,
+        "value": large["value"],
+        "unit": "site-comparisons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": large["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": large["config"],
+        "roofline": large["roofline"],
+        "wall_s_timed": large["wall_s_timed"],
+    }
+    for key in ("early_exit", "refdist_dense"):
+        if key in large:
+            result[key] = large[key]
+    if not args.no_wide:
+        result["wide"] = run_wide(args, torch, capi, workloads, np)
+    if not args.no_batched:
+        result["batched"] = run_batched(args, torch, capi, workloads, np)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(packed, args.L)

@@ -55,6 +55,18 @@ SIGNATURES = {
                                    C.c_uint32, C.c_int, C.c_int, vp, vp]),
     "hd_reference_distances_dev": (C.c_int, [vp, vp, C.c_uint64, C.c_uint64, vp, C.c_uint64,
                                              C.c_int, vp, vp]),
+    "hd_batch_create": (C.c_int, [C.POINTER(vp)]),
+    "hd_batch_destroy": (C.c_int, [vp]),
+    "hd_batch_add": (C.c_int, [vp, vp, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_uint32,
+                               u64p]),
+    "hd_batch_prepare": (C.c_int, [vp, C.c_int, C.c_int]),
+    "hd_batch_run": (C.c_int, [vp, vp]),
+    "hd_batch_run_sequential": (C.c_int, [vp, vp]),
+    "hd_batch_output": (C.c_int, [vp, C.c_uint64, vp]),
+    "hd_batch_output_dev": (C.c_int, [vp, C.c_uint64, C.POINTER(vp), u64p]),
+    "hd_batch_info": (C.c_int, [vp, u64p, u64p, u32p, intp]),
+    "hd_batch_variant_count": (C.c_int, []),
+    "hd_batch_variant_info": (C.c_int, [C.c_int, intp, intp, intp]),
     "hd_variant_count": (C.c_int, []),
     "hd_variant_info": (C.c_int, [C.c_int, intp, intp, intp, intp]),
 }
@@ -195,6 +207,72 @@ def variants() -> list[dict]:
     for v in range(lib().hd_variant_count()):
         r, c, w, p = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         check(lib().hd_variant_info(v, C.byref(r), C.byref(c), C.byref(w), C.byref(p)))
+        out.append({"variant": v, "rows_per_lane": r.value, "cols_per_wave": c.value,
+                    "waves": w.value})
+    return out
+
+
+class Batch:
+    """Many independent all-pairs datasets run as one graph-captured
+    persistent launch (hd_batch_* of the C ABI)."""
+
+    def __init__(self):
+        self._h = C.c_void_p()
+        check(lib().hd_batch_create(C.byref(self._h)))
+        self.jobs = []
+
+    def close(self):
+        if self._h:
+            lib().hd_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, packed, semantics: int, out_bytes: int, max_dist: int) -> int:
+        import numpy as np
+
+        p = np.ascontiguousarray(packed)
+        n, bps = p.shape
+        job = C.c_uint64()
+        check(lib().hd_batch_add(self._h, p.ctypes.data, n, bps, semantics, out_bytes,
+                                 max_dist, C.byref(job)))
+        self.jobs.append((n, out_bytes))
+        return job.value
+
+    def prepare(self, variant: int = -1, early_exit: bool = True):
+        check(lib().hd_batch_prepare(self._h, variant, 1 if early_exit else 0))
+
+    def run(self, stream: int = 0):
+        check(lib().hd_batch_run(self._h, stream))
+
+    def run_sequential(self, stream: int = 0):
+        check(lib().hd_batch_run_sequential(self._h, stream))
+
+    def output(self, job: int):
+        import numpy as np
+
+        n, ob = self.jobs[job]
+        out = np.zeros(n * (n - 1) // 2, np.uint8 if ob == 1 else np.uint16)
+        check(lib().hd_batch_output(self._h, job, out.ctypes.data))
+        return out
+
+    def info(self):
+        jobs, tiles, blocks, variant = C.c_uint64(), C.c_uint64(), C.c_uint32(), C.c_int()
+        check(lib().hd_batch_info(self._h, C.byref(jobs), C.byref(tiles), C.byref(blocks),
+                                  C.byref(variant)))
+        return {"jobs": jobs.value, "tiles": tiles.value, "blocks": blocks.value,
+                "variant": variant.value}
+
+
+def batch_variants() -> list[dict]:
+    out = []
+    for v in range(lib().hd_batch_variant_count()):
+        r, c, w = C.c_int(), C.c_int(), C.c_int()
+        check(lib().hd_batch_variant_info(v, C.byref(r), C.byref(c), C.byref(w)))
         out.append({"variant": v, "rows_per_lane": r.value, "cols_per_wave": c.value,
                     "waves": w.value})
     return out

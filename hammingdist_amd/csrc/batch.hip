@@ -1,0 +1,408 @@
+// Batched all-pairs: many independent datasets of uneven n and L in one
+// persistent triangular-tile launch, captured with the batched encode in a
+// hipGraph.  C ABI: hd_batch_* in include/hammingdist_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "engine_common.h"
+#include "kernels.h"
+
+namespace hd {
+
+// One dataset of a batch, as the kernels see it.
+struct BatchJobDev {
+  const uint32_t *planes;
+  void *out;
+  uint64_t tile_begin; // first index of this job in the batch's tile list
+  uint32_t n, W, P, out_bytes, max_dist, check_every;
+};
+
+struct EncodeJobDev {
+  const uint8_t *packed;
+  uint32_t *planes;
+  uint64_t bpr;
+  uint64_t block_begin; // first encode block of this job
+  uint32_t n, n_pad, W, P, runs, pad;
+  Table16 tbl;
+};
+
+// Index of the job whose range [begin[k], begin[k+1]) holds x (begin sorted).
+template <typename Job>
+__device__ __forceinline__ uint32_t find_job(const Job *__restrict__ jobs, uint32_t n_jobs,
+                                             uint64_t x, uint64_t Job::*begin) {
+  uint32_t lo = 0, hi = n_jobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (jobs[mid].*begin <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_batch_encode(const EncodeJobDev *__restrict__ jobs,
+                                                      uint32_t n_jobs) {
+  __shared__ uint8_t s_tbl[16];
+  const uint32_t j = find_job(jobs, n_jobs, blockIdx.x, &EncodeJobDev::block_begin);
+  const EncodeJobDev &J = jobs[j];
+  if (threadIdx.x < 16) s_tbl[threadIdx.x] = J.tbl.v[threadIdx.x];
+  __syncthreads();
+  const uint64_t local = blockIdx.x - J.block_begin;
+  const uint32_t row_blocks = J.n_pad / 256;
+  const uint32_t i = (uint32_t)(local % row_blocks) * 256 + threadIdx.x;
+  const uint32_t run = (uint32_t)(local / row_blocks);
+  encode_run(J.packed, J.n, J.bpr, J.W, (int)J.P, s_tbl, J.planes, i, run * kEncodeRun);
+}
+
+__global__ void k_reset_counter(unsigned long long *counter) { *counter = 0; }
+
+// Persistent workgroups take tiles from the batch's list (longest per-tile
+// work first) through one atomic counter until the list is exhausted.
+template <int R, int C, int NW>
+__global__ __launch_bounds__(kWave *NW) void k_batch_allpairs(
+    const BatchJobDev *__restrict__ jobs, uint32_t n_jobs, uint64_t total_tiles,
+    unsigned long long *__restrict__ counter) {
+  constexpr uint32_t TI = kWave * R;
+  constexpr uint32_t TJ = NW * C;
+  static_assert(TI % TJ == 0, "tile rows must be a multiple of tile columns");
+  __shared__ unsigned long long s_t;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  for (;;) {
+    if (threadIdx.x == 0) s_t = atomicAdd(counter, 1ull);
+    __syncthreads();
+    const uint64_t t = s_t;
+    __syncthreads();
+    if (t >= total_tiles) break;
+    const uint32_t j = find_job(jobs, n_jobs, t, &BatchJobDev::tile_begin);
+    const BatchJobDev &J = jobs[j];
+    uint32_t b, cb;
+    tile_coords(t - J.tile_begin, TI / TJ, b, cb);
+    const uint32_t i0 = b * TI, j0 = cb * TJ + wave * C;
+#define HD_BATCH_TILE(PP)                                                              \
+  if (J.out_bytes == 1)                                                                \
+    tile_wave<PP, R, C, uint8_t>(J.planes, J.n, J.W, i0, j0, (uint8_t *)J.out, 0,      \
+                                 J.max_dist, J.check_every);                           \
+  else                                                                                 \
+    tile_wave<PP, R, C, uint16_t>(J.planes, J.n, J.W, i0, j0, (uint16_t *)J.out, 0,    \
+                                  J.max_dist, J.check_every);
+    switch (J.P) {
+    case 4:
+      HD_BATCH_TILE(4)
+      break;
+    case 5:
+      HD_BATCH_TILE(5)
+      break;
+    default:
+      HD_BATCH_TILE(6)
+      break;
+    }
+#undef HD_BATCH_TILE
+  }
+}
+
+// Tile shapes of the batched kernel (rows per lane, columns per wave, waves).
+struct BatchVariant {
+  int R, C, NW;
+};
+static const BatchVariant kBatchVariants[] = {
+    {2, 8, 4},  // 128 x 32
+    {1, 8, 4},  // 64 x 32: the smallest datasets
+    {2, 16, 4}, // 128 x 64
+    {4, 8, 4},  // 256 x 32
+    {1, 16, 4}, // 64 x 64
+};
+static constexpr int kNumBatchVariants = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
+
+template <int R, int C, int NW> static const void *batch_kernel() {
+  return reinterpret_cast<const void *>(&k_batch_allpairs<R, C, NW>);
+}
+
+static const void *batch_kernel_of(int v) {
+  switch (v) {
+  case 0: return batch_kernel<2, 8, 4>();
+  case 1: return batch_kernel<1, 8, 4>();
+  case 2: return batch_kernel<2, 16, 4>();
+  case 3: return batch_kernel<4, 8, 4>();
+  default: return batch_kernel<1, 16, 4>();
+  }
+}
+
+} // namespace hd
+
+using namespace hd;
+
+// One batch: per-job device buffers, the job tables and the captured graph.
+struct hd_batch {
+  struct Job {
+    uint64_t n = 0, bpr = 0;
+    int semantics = HD_SEM_DENSE, out_bytes = 1;
+    uint32_t max_dist = 0, mask = 0xFFFF;
+    Geometry g;
+    DeviceBuffer packed, planes, out;
+  };
+  std::vector<std::unique_ptr<Job>> jobs;
+  DeviceBuffer d_jobs, d_enc, d_counter;
+  std::vector<BatchJobDev> host_jobs;
+  uint32_t n_jobs_active = 0;
+  uint64_t total_tiles = 0, enc_blocks = 0;
+  int variant = -1, early_exit = 1;
+  uint32_t persistent_blocks = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool prepared = false;
+
+  ~hd_batch() { drop_graph(); }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    prepared = false;
+  }
+};
+
+namespace {
+
+// Enqueues the batched encode, the counter reset and the persistent kernel.
+void enqueue_batch(hd_batch *b, hipStream_t s) {
+  if (b->enc_blocks) {
+    HD_REQUIRE(b->enc_blocks < (1ull << 31), "batch too large for one encode launch");
+    hipLaunchKernelGGL(k_batch_encode, dim3((uint32_t)b->enc_blocks), dim3(256), 0, s,
+                       b->d_enc.get<EncodeJobDev>(), (uint32_t)b->jobs.size());
+    HD_HIP(hipGetLastError());
+  }
+  if (!b->total_tiles) return;
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s,
+                     b->d_counter.get<unsigned long long>());
+  HD_HIP(hipGetLastError());
+  const BatchVariant &v = kBatchVariants[b->variant];
+  const BatchJobDev *jp = b->d_jobs.get<BatchJobDev>();
+  uint32_t nj = b->n_jobs_active;
+  uint64_t tt = b->total_tiles;
+  unsigned long long *ctr = b->d_counter.get<unsigned long long>();
+  void *args[] = {(void *)&jp, (void *)&nj, (void *)&tt, (void *)&ctr};
+  HD_HIP(hipLaunchKernel(batch_kernel_of(b->variant), dim3(b->persistent_blocks),
+                         dim3(kWave * v.NW), args, 0, s));
+}
+
+void prepare(hd_batch *b, int variant, int early_exit) {
+  HD_REQUIRE(!b->jobs.empty(), "empty batch");
+  HD_REQUIRE(variant >= -1 && variant < kNumBatchVariants, "unknown batch variant");
+  b->drop_graph();
+  b->variant = variant < 0 ? 0 : variant;
+  b->early_exit = early_exit;
+  const BatchVariant &v = kBatchVariants[b->variant];
+  const uint64_t TI = 64ull * v.R, TJ = (uint64_t)v.NW * v.C, M = TI / TJ;
+  hipStream_t s = lib_stream();
+
+  // Tiles of the longest rows first (LPT order); every job keeps its own
+  // output buffer.
+  std::vector<size_t> order(b->jobs.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    return (uint64_t)b->jobs[x]->g.W * b->jobs[x]->g.code.planes >
+           (uint64_t)b->jobs[y]->g.W * b->jobs[y]->g.code.planes;
+  });
+  b->host_jobs.clear();
+  std::vector<EncodeJobDev> enc;
+  uint64_t tiles = 0, blocks = 0;
+  for (size_t k : order) {
+    auto &J = *b->jobs[k];
+    EncodeJobDev e{};
+    e.packed = J.packed.get<uint8_t>();
+    e.planes = J.planes.get<uint32_t>();
+    e.bpr = J.bpr;
+    e.block_begin = blocks;
+    e.n = (uint32_t)J.n;
+    e.n_pad = (uint32_t)J.g.n_pad;
+    e.W = J.g.W;
+    e.P = (uint32_t)J.g.code.planes;
+    e.runs = (J.g.W + kEncodeRun - 1) / kEncodeRun;
+    std::memcpy(e.tbl.v, J.g.code.table, 16);
+    blocks += (uint64_t)(e.n_pad / 256) * e.runs;
+    enc.push_back(e);
+    if (J.n < 2) continue;
+    BatchJobDev d{};
+    d.planes = J.planes.get<uint32_t>();
+    d.out = J.out.get();
+    d.tile_begin = tiles;
+    d.n = (uint32_t)J.n;
+    d.W = J.g.W;
+    d.P = (uint32_t)J.g.code.planes;
+    d.out_bytes = (uint32_t)J.out_bytes;
+    d.max_dist = J.max_dist;
+    d.check_every = check_interval(early_exit != 0, J.max_dist, J.g.W);
+    const uint64_t nb = (J.n + TI - 1) / TI;
+    tiles += tiles_before(nb, M);
+    b->host_jobs.push_back(d);
+  }
+  b->total_tiles = tiles;
+  b->enc_blocks = blocks;
+  b->n_jobs_active = (uint32_t)b->host_jobs.size();
+  b->d_jobs.alloc(std::max<size_t>(1, b->host_jobs.size()) * sizeof(BatchJobDev));
+  b->d_enc.alloc(enc.size() * sizeof(EncodeJobDev));
+  b->d_counter.alloc(sizeof(unsigned long long));
+  if (!b->host_jobs.empty())
+    HD_HIP(hipMemcpyAsync(b->d_jobs.get(), b->host_jobs.data(),
+                          b->host_jobs.size() * sizeof(BatchJobDev), hipMemcpyHostToDevice, s));
+  HD_HIP(hipMemcpyAsync(b->d_enc.get(), enc.data(), enc.size() * sizeof(EncodeJobDev),
+                        hipMemcpyHostToDevice, s));
+  HD_HIP(hipStreamSynchronize(s));
+
+  // Enough workgroups to fill every CU at the kernel's occupancy.
+  int dev = 0, cus = 0, per_cu = 0;
+  HD_HIP(hipGetDevice(&dev));
+  HD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_of(b->variant),
+                                                      kWave * v.NW, 0));
+  b->persistent_blocks = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>((uint64_t)cus * std::max(per_cu, 1), tiles));
+
+  // Capture: batched encode -> counter reset (a kernel, not a memset) ->
+  // persistent all-pairs.
+  Stream cap;
+  HD_HIP(hipStreamBeginCapture(cap.get(), hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue_batch(b, cap.get());
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(cap.get(), &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HD_HIP(hipStreamEndCapture(cap.get(), &b->graph));
+  HD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->prepared = true;
+}
+
+} // namespace
+
+extern "C" {
+
+int hd_batch_create(hd_batch **out) {
+  return guarded([&] {
+    HD_REQUIRE(out, "null output");
+    *out = new hd_batch();
+  });
+}
+
+int hd_batch_destroy(hd_batch *b) {
+  return guarded([&] { delete b; });
+}
+
+int hd_batch_add(hd_batch *b, const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
+                 int semantics, int out_bytes, uint32_t max_dist, uint64_t *job) {
+  return guarded([&] {
+    HD_REQUIRE(b && job, "null batch or job output");
+    HD_REQUIRE(out_bytes == 1 || out_bytes == 2, "batch jobs store uint8 or uint16 cells");
+    HD_REQUIRE(max_dist <= (out_bytes == 1 ? 255u : 65535u),
+               "max_dist does not fit the output cell type");
+    HD_REQUIRE(b->jobs.size() < 65536, "too many jobs in one batch");
+    HD_REQUIRE(n == 0 || packed, "null buffer");
+    ensure_device();
+    geometry(n, bytes_per_seq, HD_SEM_DENSE, 0); // argument checks
+    hipStream_t s = lib_stream();
+    auto J = std::make_unique<hd_batch::Job>();
+    J->n = n;
+    J->bpr = bytes_per_seq;
+    J->semantics = semantics;
+    J->out_bytes = out_bytes;
+    J->max_dist = max_dist;
+    J->packed.alloc(std::max<uint64_t>(n * bytes_per_seq, 8));
+    if (n)
+      HD_HIP(hipMemcpyAsync(J->packed.get(), packed, n * bytes_per_seq, hipMemcpyHostToDevice, s));
+    if (semantics != HD_SEM_DENSE && n) {
+      DeviceBuffer d_mask(4);
+      launch_nibble_mask(J->packed.get<uint8_t>(), n * bytes_per_seq, d_mask.get<uint32_t>(), s);
+      HD_HIP(hipMemcpyAsync(&J->mask, d_mask.get(), 4, hipMemcpyDeviceToHost, s));
+      HD_HIP(hipStreamSynchronize(s));
+    }
+    J->g = geometry(n, bytes_per_seq, semantics, J->mask);
+    J->planes.alloc(J->g.total_words() * 4);
+    J->out.alloc(std::max<uint64_t>(tri_cells(n) * out_bytes, 1));
+    HD_HIP(hipStreamSynchronize(s));
+    *job = b->jobs.size();
+    b->jobs.push_back(std::move(J));
+    b->prepared = false;
+  });
+}
+
+int hd_batch_prepare(hd_batch *b, int variant, int early_exit) {
+  return guarded([&] {
+    HD_REQUIRE(b, "null batch");
+    prepare(b, variant, early_exit);
+  });
+}
+
+int hd_batch_run(hd_batch *b, void *stream) {
+  return guarded([&] {
+    HD_REQUIRE(b && b->prepared, "batch not prepared");
+    HD_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  });
+}
+
+int hd_batch_run_sequential(hd_batch *b, void *stream) {
+  return guarded([&] {
+    HD_REQUIRE(b, "null batch");
+    hipStream_t s = (hipStream_t)stream;
+    for (auto &Jp : b->jobs) {
+      auto &J = *Jp;
+      if (J.n == 0) continue;
+      launch_encode(J.g, J.packed.get<uint8_t>(), J.planes.get<uint32_t>(), s);
+      if (J.n < 2) continue;
+      const int v = default_variant(J.g.code.planes, J.n);
+      const uint64_t TI = 64ull * variant_of(v).R;
+      run_allpairs(J.g, J.planes.get<uint32_t>(), J.out_bytes, J.max_dist, b->early_exit != 0,
+                   v, 0, (J.n + TI - 1) / TI, J.out.get(), 0, s);
+    }
+  });
+}
+
+int hd_batch_output(hd_batch *b, uint64_t job, void *host_out) {
+  return guarded([&] {
+    HD_REQUIRE(b && job < b->jobs.size(), "unknown job");
+    auto &J = *b->jobs[job];
+    const uint64_t bytes = tri_cells(J.n) * J.out_bytes;
+    if (!bytes) return;
+    HD_REQUIRE(host_out, "null buffer");
+    hipStream_t s = lib_stream();
+    HD_HIP(hipMemcpyAsync(host_out, J.out.get(), bytes, hipMemcpyDeviceToHost, s));
+    HD_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int hd_batch_output_dev(hd_batch *b, uint64_t job, void **d_cells, uint64_t *cells) {
+  return guarded([&] {
+    HD_REQUIRE(b && job < b->jobs.size() && d_cells && cells, "unknown job or null output");
+    *d_cells = b->jobs[job]->out.get();
+    *cells = tri_cells(b->jobs[job]->n);
+  });
+}
+
+int hd_batch_info(hd_batch *b, uint64_t *jobs, uint64_t *tiles, uint32_t *blocks,
+                  int *variant) {
+  return guarded([&] {
+    HD_REQUIRE(b, "null batch");
+    if (jobs) *jobs = b->jobs.size();
+    if (tiles) *tiles = b->total_tiles;
+    if (blocks) *blocks = b->persistent_blocks;
+    if (variant) *variant = b->variant;
+  });
+}
+
+int hd_batch_variant_count(void) { return kNumBatchVariants; }
+
+int hd_batch_variant_info(int variant, int *rows_per_lane, int *cols_per_wave, int *waves) {
+  return guarded([&] {
+    HD_REQUIRE(variant >= 0 && variant < kNumBatchVariants, "unknown batch variant");
+    if (rows_per_lane) *rows_per_lane = kBatchVariants[variant].R;
+    if (cols_per_wave) *cols_per_wave = kBatchVariants[variant].C;
+    if (waves) *waves = kBatchVariants[variant].NW;
+  });
+}
+
+} // extern "C"

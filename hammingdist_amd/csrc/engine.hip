@@ -85,9 +85,6 @@ PlaneCode plane_code(int semantics, uint32_t mask) {
 // ---------------------------------------------------------------------------
 // tile variants of the all-pairs kernel
 // ---------------------------------------------------------------------------
-struct Variant {
-  int R, C, NW;
-};
 // Rows per lane, columns per wave, waves per workgroup.
 static const Variant kVariants[] = {
     {4, 16, 4}, // 256 x 64 tiles: the large-n shape for 4 planes
@@ -98,7 +95,10 @@ static const Variant kVariants[] = {
 };
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
-static int default_variant(int planes, uint64_t n) {
+int variant_count() { return kNumVariants; }
+const Variant &variant_of(int v) { return kVariants[v]; }
+
+int default_variant(int planes, uint64_t n) {
   if (n < 512) return 4;
   if (planes == 4) return n < 2048 ? 3 : 0;
   return 2;
@@ -166,17 +166,9 @@ static void launch_allpairs_o(int P, int v, const uint32_t *planes, uint32_t n,
   }
 }
 
-// Geometry of one encoded dataset.
-struct Geometry {
-  uint64_t n = 0, bpr = 0, n_pad = 0;
-  uint32_t W = 0; // 32-site words per row
-  PlaneCode code;
-  uint64_t total_words() const { return (uint64_t)W * code.planes * n_pad; }
-};
-
 static constexpr uint64_t kRowPad = 512; // multiple of every variant's TI
 
-static Geometry geometry(uint64_t n, uint64_t bpr, int semantics, uint32_t mask) {
+Geometry geometry(uint64_t n, uint64_t bpr, int semantics, uint32_t mask) {
   HD_REQUIRE(bpr > 0 && bpr % 8 == 0,
              "bytes_per_seq must be a positive multiple of 8");
   HD_REQUIRE(n < (1ull << 31), "too many sequences");
@@ -191,15 +183,17 @@ static Geometry geometry(uint64_t n, uint64_t bpr, int semantics, uint32_t mask)
 
 static uint32_t tile_rows(int v) { return 64u * kVariants[v].R; }
 
-static void run_allpairs(const Geometry &g, const uint32_t *planes, int out_bytes,
-                         uint32_t max_dist, bool early_exit, int variant,
-                         uint64_t rb0, uint64_t rb1, void *out,
-                         uint64_t cell_offset, hipStream_t s) {
-  const uint32_t W = g.W;
+uint32_t check_interval(bool early_exit, uint32_t max_dist, uint32_t W) {
   // The early-exit vote runs every 32 words (1024 sites); it is skipped when
   // max_dist is out of reach.
-  const uint32_t check_every =
-      (early_exit && (uint64_t)max_dist < 32ull * W) ? 32u : W;
+  return (early_exit && (uint64_t)max_dist < 32ull * W) ? 32u : W;
+}
+
+void run_allpairs(const Geometry &g, const uint32_t *planes, int out_bytes,
+                  uint32_t max_dist, bool early_exit, int variant, uint64_t rb0,
+                  uint64_t rb1, void *out, uint64_t cell_offset, hipStream_t s) {
+  const uint32_t W = g.W;
+  const uint32_t check_every = check_interval(early_exit, max_dist, W);
   const int P = g.code.planes;
   switch (out_bytes) {
   case 1:
@@ -236,8 +230,6 @@ static void check_max_dist(int out_bytes, uint32_t max_dist) {
 // row and writes 8 x P plane words; the 64 rows of a wavefront form one row
 // group, so each store is 256 contiguous bytes.  Rows >= n and sites past the row end encode as '-' (all
 // planes set), which never counts against anything.
-static constexpr int kEncodeRun = 8;
-
 __global__ __launch_bounds__(256) void k_encode(const uint8_t *__restrict__ packed,
                                                 uint32_t n, uint64_t bpr,
                                                 uint32_t n_pad, uint32_t W,
@@ -248,43 +240,11 @@ __global__ __launch_bounds__(256) void k_encode(const uint8_t *__restrict__ pack
   __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_pad) return;
-  const uint32_t k0 = blockIdx.y * kEncodeRun;
-  const uint8_t *row = packed + (uint64_t)i * bpr;
-  for (int kk = 0; kk < kEncodeRun; ++kk) {
-    const uint32_t k = k0 + kk;
-    if (k >= W) break;
-    uint32_t w[6] = {0, 0, 0, 0, 0, 0};
-    const uint64_t off = (uint64_t)k * 16;
-    uint32_t bytes[4];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint64_t o = off + 8 * h;
-      if (i < n && o + 8 <= bpr) {
-        const uint2 u = *reinterpret_cast<const uint2 *>(row + o);
-        bytes[2 * h] = u.x;
-        bytes[2 * h + 1] = u.y;
-      } else {
-        bytes[2 * h] = 0xFFFFFFFFu;
-        bytes[2 * h + 1] = 0xFFFFFFFFu;
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < 32; ++s) {
-      const uint32_t nib = (bytes[s >> 3] >> (4 * (s & 7))) & 15u;
-      const uint32_t m = s_tbl[nib];
-#pragma unroll
-      for (int p = 0; p < 6; ++p)
-        if (p < P) w[p] |= ((m >> p) & 1u) << s;
-    }
-    uint32_t *dst = planes + (((uint64_t)(i / 64) * W + k) * P) * 64 + (i % 64);
-#pragma unroll
-    for (int p = 0; p < 6; ++p)
-      if (p < P) dst[p * 64] = w[p];
-  }
+  encode_run(packed, n, bpr, W, P, s_tbl, planes, i, blockIdx.y * kEncodeRun);
 }
 
-static void launch_encode(const Geometry &g, const uint8_t *d_packed,
-                          uint32_t *d_planes, hipStream_t s) {
+void launch_encode(const Geometry &g, const uint8_t *d_packed, uint32_t *d_planes,
+                   hipStream_t s) {
   Table16 t;
   std::memcpy(t.v, g.code.table, 16);
   const uint32_t runs = (g.W + kEncodeRun - 1) / kEncodeRun;
@@ -317,8 +277,8 @@ __global__ __launch_bounds__(256) void k_nibble_mask(const uint2 *__restrict__ d
   if ((threadIdx.x & 63) == 0) atomicOr(mask, m);
 }
 
-static void launch_nibble_mask(const uint8_t *d_packed, uint64_t bytes,
-                               uint32_t *d_mask, hipStream_t s) {
+void launch_nibble_mask(const uint8_t *d_packed, uint64_t bytes, uint32_t *d_mask,
+                        hipStream_t s) {
   const uint64_t units = bytes / 8;
   HD_HIP(hipMemsetAsync(d_mask, 0, sizeof(uint32_t), s));
   if (!units) return;
@@ -560,7 +520,7 @@ static uint64_t emit_t(const T *d_cells, uint64_t n_cells, uint32_t thr,
 // ---------------------------------------------------------------------------
 // library stream
 // ---------------------------------------------------------------------------
-static hipStream_t lib_stream() {
+hipStream_t lib_stream() {
   static std::once_flag once;
   static hipStream_t s = nullptr;
   std::call_once(once, [] {

@@ -143,4 +143,33 @@ PlaneCode plane_code(int semantics, uint32_t nibble_mask);
 
 inline uint64_t tri_cells(uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; }
 
+// Geometry of one encoded dataset (planes[g][k][p][64], see kernels.h).
+struct Geometry {
+  uint64_t n = 0, bpr = 0, n_pad = 0;
+  uint32_t W = 0; // 32-site words per row
+  PlaneCode code;
+  uint64_t total_words() const { return (uint64_t)W * code.planes * n_pad; }
+};
+Geometry geometry(uint64_t n, uint64_t bpr, int semantics, uint32_t mask);
+
+// Tile shapes of the all-pairs kernels: rows per lane R, columns per wave C,
+// waves per workgroup NW; a tile is 64 R rows by NW C columns.
+struct Variant {
+  int R, C, NW;
+};
+int variant_count();
+const Variant &variant_of(int v);
+int default_variant(int planes, uint64_t n);
+
+void launch_encode(const Geometry &g, const uint8_t *d_packed, uint32_t *d_planes,
+                   hipStream_t s);
+void launch_nibble_mask(const uint8_t *d_packed, uint64_t bytes, uint32_t *d_mask,
+                        hipStream_t s);
+// All-pairs triangle of row blocks [rb0, rb1) (plain, one launch).
+void run_allpairs(const Geometry &g, const uint32_t *planes, int out_bytes,
+                  uint32_t max_dist, bool early_exit, int variant, uint64_t rb0,
+                  uint64_t rb1, void *out, uint64_t cell_offset, hipStream_t s);
+uint32_t check_interval(bool early_exit, uint32_t max_dist, uint32_t W);
+hipStream_t lib_stream();
+
 } // namespace hd

@@ -38,34 +38,26 @@ struct Table16 {
 // and per 32 sites the work is P-1 AND/AND-OR ops, one NOR-with-AND and one
 // popcount-accumulate: 5 ops for the 4-plane dense code.
 // ---------------------------------------------------------------------------
-template <int P, int R, int C, int NW, typename OutT>
-__global__ __launch_bounds__(kWave *NW) void k_allpairs(
-    const uint32_t *__restrict__ planes, uint32_t n, uint32_t W, uint64_t tile_begin, uint32_t n_tiles, OutT *__restrict__ out,
-    uint64_t cell_offset, uint32_t max_dist, uint32_t check_every) {
-  constexpr uint32_t TI = kWave * R;
-  constexpr uint32_t TJ = NW * C;
-  constexpr uint32_t M = TI / TJ;
-  static_assert(TI % TJ == 0, "tile rows must be a multiple of tile columns");
+// Tile t -> (row block b, column block cb) of a triangle tiled with M column
+// blocks per row block: row block b holds tiles [M b(b+1)/2, M (b+1)(b+2)/2).
+__device__ __forceinline__ void tile_coords(uint64_t t, uint32_t M, uint32_t &b,
+                                            uint32_t &cb) {
+  uint64_t r = (uint64_t)((sqrt(8.0 * (double)t / (double)M + 1.0) - 1.0) * 0.5);
+  while ((uint64_t)M * (r + 1) * (r + 2) / 2 <= t) ++r;
+  while (r > 0 && (uint64_t)M * r * (r + 1) / 2 > t) --r;
+  b = (uint32_t)r;
+  cb = (uint32_t)(t - (uint64_t)M * r * (r + 1) / 2);
+}
 
-  // The dispatcher hands consecutive workgroups to the 8 XCDs in turn; map
-  // them so that each XCD walks a contiguous run of tiles (same row block,
-  // neighbouring column blocks) and shares the row data in its own L2.
-  const uint32_t g = blockIdx.x;
-  const uint32_t q8 = n_tiles >> 3, r8 = n_tiles & 7u, x = g & 7u;
-  const uint64_t t =
-      tile_begin + (uint64_t)x * q8 + (x < r8 ? x : r8) + (g >> 3);
-
-  // Tile t -> (row block b, column block cb); row block b has M (b+1) tiles.
-  uint64_t b = (uint64_t)((sqrt(8.0 * (double)t / (double)M + 1.0) - 1.0) * 0.5);
-  while ((uint64_t)M * (b + 1) * (b + 2) / 2 <= t) ++b;
-  while (b > 0 && (uint64_t)M * b * (b + 1) / 2 > t) --b;
-  const uint32_t cb = (uint32_t)(t - (uint64_t)M * b * (b + 1) / 2);
-
+// One wave's share of a tile: rows i0 + 64 q + lane (q < R) against columns
+// j0 .. j0 + C - 1, stored into the triangle at cell i(i-1)/2 + j - cell_offset.
+template <int P, int R, int C, typename OutT>
+__device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, uint32_t n,
+                                          uint32_t W, uint32_t i0, uint32_t j0,
+                                          OutT *__restrict__ out, uint64_t cell_offset,
+                                          uint32_t max_dist, uint32_t check_every) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const uint32_t i0 = (uint32_t)b * TI;
-  const uint32_t j0 = cb * TJ + wave * C;
-  const uint32_t i_end = min(i0 + TI, n);
+  const uint32_t i_end = min(i0 + (uint32_t)(kWave * R), n);
   if (j0 + 1 >= i_end) return; // no pair j < i inside this wave's block
 
   // Pairs outside the triangle start saturated so that the early-exit vote
@@ -129,6 +121,76 @@ __global__ __launch_bounds__(kWave *NW) void k_allpairs(
       for (int c = 0; c < C; ++c)
         if (j0 + c < i) row[c] = (OutT)min(acc[q][c], max_dist);
     }
+  }
+}
+
+template <int P, int R, int C, int NW, typename OutT>
+__global__ __launch_bounds__(kWave *NW) void k_allpairs(
+    const uint32_t *__restrict__ planes, uint32_t n, uint32_t W, uint64_t tile_begin,
+    uint32_t n_tiles, OutT *__restrict__ out, uint64_t cell_offset, uint32_t max_dist,
+    uint32_t check_every) {
+  constexpr uint32_t TI = kWave * R;
+  constexpr uint32_t TJ = NW * C;
+  static_assert(TI % TJ == 0, "tile rows must be a multiple of tile columns");
+
+  // The dispatcher hands consecutive workgroups to the 8 XCDs in turn; map
+  // them so that each XCD walks a contiguous run of tiles (same row block,
+  // neighbouring column blocks) and shares the row data in its own L2.
+  const uint32_t g = blockIdx.x;
+  const uint32_t q8 = n_tiles >> 3, r8 = n_tiles & 7u, x = g & 7u;
+  const uint64_t t = tile_begin + (uint64_t)x * q8 + (x < r8 ? x : r8) + (g >> 3);
+  uint32_t b, cb;
+  tile_coords(t, TI / TJ, b, cb);
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  tile_wave<P, R, C, OutT>(planes, n, W, b * TI, cb * TJ + wave * C, out, cell_offset,
+                           max_dist, check_every);
+}
+
+// ---------------------------------------------------------------------------
+// encode: packed nibble rows -> bit-sliced planes
+// ---------------------------------------------------------------------------
+// Row i, words k0 .. k0+7: reads 128 contiguous bytes of the row and writes
+// 8 x P plane words; the 64 rows of a wavefront form one row group, so each
+// store is 256 contiguous bytes.  Rows >= n and sites past the row end
+// encode as '-' (table entry 15, all planes set), which never counts.
+constexpr int kEncodeRun = 8;
+
+__device__ __forceinline__ void encode_run(const uint8_t *__restrict__ packed, uint32_t n,
+                                           uint64_t bpr, uint32_t W, int P,
+                                           const uint8_t *s_tbl,
+                                           uint32_t *__restrict__ planes, uint32_t i,
+                                           uint32_t k0) {
+  const uint8_t *row = packed + (uint64_t)i * bpr;
+  for (int kk = 0; kk < kEncodeRun; ++kk) {
+    const uint32_t k = k0 + kk;
+    if (k >= W) break;
+    uint32_t w[6] = {0, 0, 0, 0, 0, 0};
+    const uint64_t off = (uint64_t)k * 16;
+    uint32_t bytes[4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint64_t o = off + 8 * h;
+      if (i < n && o + 8 <= bpr) {
+        const uint2 u = *reinterpret_cast<const uint2 *>(row + o);
+        bytes[2 * h] = u.x;
+        bytes[2 * h + 1] = u.y;
+      } else {
+        bytes[2 * h] = 0xFFFFFFFFu;
+        bytes[2 * h + 1] = 0xFFFFFFFFu;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 32; ++s) {
+      const uint32_t nib = (bytes[s >> 3] >> (4 * (s & 7))) & 15u;
+      const uint32_t m = s_tbl[nib];
+#pragma unroll
+      for (int p = 0; p < 6; ++p)
+        if (p < P) w[p] |= ((m >> p) & 1u) << s;
+    }
+    uint32_t *dst = planes + (((uint64_t)(i / 64) * W + k) * P) * 64 + (i % 64);
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+      if (p < P) dst[p * 64] = w[p];
   }
 }
 

@@ -165,6 +165,36 @@ int hd_reference_distances_dev(const uint8_t *d_query, const uint8_t *d_db,
                                const uint64_t *d_rows, uint64_t m,
                                int semantics, uint32_t *d_out, void *stream);
 
+/* ---- batched all-pairs (no counterpart in the reference) ---------------- */
+/* Many independent datasets of uneven n and L computed by one persistent
+ * triangular-tile launch (captured with the batched encode in a hipGraph);
+ * the per-dataset result equals hd_all_pairs on that dataset.              */
+typedef struct hd_batch hd_batch;
+int hd_batch_create(hd_batch **out);
+int hd_batch_destroy(hd_batch *batch);
+/* Uploads one dataset (n packed rows); *job receives its index.  out_bytes
+ * is 1 or 2; max_dist must fit the cell type.                               */
+int hd_batch_add(hd_batch *batch, const uint8_t *packed, uint64_t n,
+                 uint64_t bytes_per_seq, int semantics, int out_bytes,
+                 uint32_t max_dist, uint64_t *job);
+/* Builds the tile list with tile shape `variant` (see hd_batch_variant_info;
+ * -1 = default) and captures the graph.                                     */
+int hd_batch_prepare(hd_batch *batch, int variant, int early_exit);
+/* Replays the captured graph on `stream` (asynchronous).                    */
+int hd_batch_run(hd_batch *batch, void *stream);
+/* The same work as plain launches, dataset after dataset (the baseline).    */
+int hd_batch_run_sequential(hd_batch *batch, void *stream);
+/* Copies one job's triangle to host memory (synchronous).                   */
+int hd_batch_output(hd_batch *batch, uint64_t job, void *host_out);
+/* Device address and cell count of one job's triangle.                      */
+int hd_batch_output_dev(hd_batch *batch, uint64_t job, void **d_cells,
+                        uint64_t *cells);
+int hd_batch_info(hd_batch *batch, uint64_t *jobs, uint64_t *tiles,
+                  uint32_t *blocks, int *variant);
+int hd_batch_variant_count(void);
+int hd_batch_variant_info(int variant, int *rows_per_lane, int *cols_per_wave,
+                          int *waves);
+
 /* Number of all-pairs tile shapes compiled in, and a description of one.    */
 int hd_variant_count(void);
 int hd_variant_info(int variant, int *rows_per_lane, int *cols_per_wave,
