@@ -14,6 +14,16 @@
 
 namespace hd {
 
+// Tile shapes of the batched kernel (rows per lane R, columns per wave C; 4
+// waves per workgroup, so a tile is 64 R rows by 4 C columns).  One launch
+// uses one shape for every job; the autotuner picks it per batch.
+constexpr int kBatchWaves = 4;
+struct Shape {
+  int R, C;
+};
+constexpr Shape kShapes[] = {{4, 16}, {4, 8}, {2, 8}, {2, 16}, {1, 8}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
 // One dataset of a batch, as the kernels see it.
 struct BatchJobDev {
   const uint32_t *planes;
@@ -60,14 +70,17 @@ __global__ __launch_bounds__(256) void k_batch_encode(const EncodeJobDev *__rest
 
 __global__ void k_reset_counter(unsigned long long *counter) { *counter = 0; }
 
-// Persistent workgroups take tiles from the batch's list (longest per-tile
-// work first) through one atomic counter until the list is exhausted.
-template <int R, int C, int NW>
-__global__ __launch_bounds__(kWave *NW) void k_batch_allpairs(
+// Persistent workgroups take tiles from the batch's list (largest per-tile
+// work first) through one atomic counter until the list is exhausted; each
+// tile is computed with its job's plane count and cell type.  ONLY4: every
+// job has 4 planes (the dense code), so only that tile loop is compiled in
+// and the kernel keeps the register budget of the plain kernel.
+template <int R, int C, bool ONLY4>
+__global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_allpairs(
     const BatchJobDev *__restrict__ jobs, uint32_t n_jobs, uint64_t total_tiles,
     unsigned long long *__restrict__ counter) {
   constexpr uint32_t TI = kWave * R;
-  constexpr uint32_t TJ = NW * C;
+  constexpr uint32_t TJ = kBatchWaves * C;
   static_assert(TI % TJ == 0, "tile rows must be a multiple of tile columns");
   __shared__ unsigned long long s_t;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -82,54 +95,49 @@ __global__ __launch_bounds__(kWave *NW) void k_batch_allpairs(
     uint32_t b, cb;
     tile_coords(t - J.tile_begin, TI / TJ, b, cb);
     const uint32_t i0 = b * TI, j0 = cb * TJ + wave * C;
-#define HD_BATCH_TILE(PP)                                                              \
-  if (J.out_bytes == 1)                                                                \
-    tile_wave<PP, R, C, uint8_t>(J.planes, J.n, J.W, i0, j0, (uint8_t *)J.out, 0,      \
-                                 J.max_dist, J.check_every);                           \
-  else                                                                                 \
-    tile_wave<PP, R, C, uint16_t>(J.planes, J.n, J.W, i0, j0, (uint16_t *)J.out, 0,    \
-                                  J.max_dist, J.check_every);
-    switch (J.P) {
+    // 5 and 6 planes with 16 columns per wave would need more column words
+    // than the SGPR file holds: such tiles run as two passes of 8 columns.
+    constexpr int CW = C > 8 ? 8 : C;
+#define HD_BATCH_TILE(PP, CC, JJ)                                                        \
+  tile_wave<PP, R, CC>(J.planes, J.n, J.W, i0, JJ, J.out, J.out_bytes, 0, J.max_dist,     \
+                       J.check_every);
+    switch (ONLY4 ? 4u : J.P) {
     case 4:
-      HD_BATCH_TILE(4)
+      HD_BATCH_TILE(4, C, j0)
       break;
     case 5:
-      HD_BATCH_TILE(5)
+      for (uint32_t h = 0; h < (uint32_t)(C / CW); ++h) {
+        HD_BATCH_TILE(5, CW, j0 + h * CW)
+      }
       break;
     default:
-      HD_BATCH_TILE(6)
+      for (uint32_t h = 0; h < (uint32_t)(C / CW); ++h) {
+        HD_BATCH_TILE(6, CW, j0 + h * CW)
+      }
       break;
     }
 #undef HD_BATCH_TILE
   }
 }
 
-// Tile shapes of the batched kernel (rows per lane, columns per wave, waves).
-struct BatchVariant {
-  int R, C, NW;
-};
-static const BatchVariant kBatchVariants[] = {
-    {2, 8, 4},  // 128 x 32
-    {1, 8, 4},  // 64 x 32: the smallest datasets
-    {2, 16, 4}, // 128 x 64
-    {4, 8, 4},  // 256 x 32
-    {1, 16, 4}, // 64 x 64
-};
-static constexpr int kNumBatchVariants = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
-
-template <int R, int C, int NW> static const void *batch_kernel() {
-  return reinterpret_cast<const void *>(&k_batch_allpairs<R, C, NW>);
+template <int V> static const void *batch_kernel_v(bool only4) {
+  return only4 ? reinterpret_cast<const void *>(
+                     &k_batch_allpairs<kShapes[V].R, kShapes[V].C, true>)
+               : reinterpret_cast<const void *>(
+                     &k_batch_allpairs<kShapes[V].R, kShapes[V].C, false>);
 }
 
-static const void *batch_kernel_of(int v) {
+static const void *batch_kernel_of(int v, bool only4) {
   switch (v) {
-  case 0: return batch_kernel<2, 8, 4>();
-  case 1: return batch_kernel<1, 8, 4>();
-  case 2: return batch_kernel<2, 16, 4>();
-  case 3: return batch_kernel<4, 8, 4>();
-  default: return batch_kernel<1, 16, 4>();
+  case 0: return batch_kernel_v<0>(only4);
+  case 1: return batch_kernel_v<1>(only4);
+  case 2: return batch_kernel_v<2>(only4);
+  case 3: return batch_kernel_v<3>(only4);
+  default: return batch_kernel_v<4>(only4);
   }
 }
+
+static constexpr int kNumBatchVariants = kNumShapes;
 
 } // namespace hd
 
@@ -150,6 +158,7 @@ struct hd_batch {
   uint32_t n_jobs_active = 0;
   uint64_t total_tiles = 0, enc_blocks = 0;
   int variant = -1, early_exit = 1;
+  bool only4 = true; // every job encodes to 4 planes
   uint32_t persistent_blocks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -179,14 +188,13 @@ void enqueue_batch(hd_batch *b, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s,
                      b->d_counter.get<unsigned long long>());
   HD_HIP(hipGetLastError());
-  const BatchVariant &v = kBatchVariants[b->variant];
   const BatchJobDev *jp = b->d_jobs.get<BatchJobDev>();
   uint32_t nj = b->n_jobs_active;
   uint64_t tt = b->total_tiles;
   unsigned long long *ctr = b->d_counter.get<unsigned long long>();
   void *args[] = {(void *)&jp, (void *)&nj, (void *)&tt, (void *)&ctr};
-  HD_HIP(hipLaunchKernel(batch_kernel_of(b->variant), dim3(b->persistent_blocks),
-                         dim3(kWave * v.NW), args, 0, s));
+  HD_HIP(hipLaunchKernel(batch_kernel_of(b->variant, b->only4), dim3(b->persistent_blocks),
+                         dim3(kWave * kBatchWaves), args, 0, s));
 }
 
 void prepare(hd_batch *b, int variant, int early_exit) {
@@ -195,17 +203,17 @@ void prepare(hd_batch *b, int variant, int early_exit) {
   b->drop_graph();
   b->variant = variant < 0 ? 0 : variant;
   b->early_exit = early_exit;
-  const BatchVariant &v = kBatchVariants[b->variant];
-  const uint64_t TI = 64ull * v.R, TJ = (uint64_t)v.NW * v.C, M = TI / TJ;
   hipStream_t s = lib_stream();
+  const Shape sh = kShapes[b->variant];
+  const uint64_t TI = 64ull * sh.R, TJ = (uint64_t)kBatchWaves * sh.C;
+  auto tile_work = [&](const hd_batch::Job &J) { return (uint64_t)J.g.W * J.g.code.planes; };
 
   // Tiles of the longest rows first (LPT order); every job keeps its own
   // output buffer.
   std::vector<size_t> order(b->jobs.size());
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-    return (uint64_t)b->jobs[x]->g.W * b->jobs[x]->g.code.planes >
-           (uint64_t)b->jobs[y]->g.W * b->jobs[y]->g.code.planes;
+    return tile_work(*b->jobs[x]) > tile_work(*b->jobs[y]);
   });
   b->host_jobs.clear();
   std::vector<EncodeJobDev> enc;
@@ -236,10 +244,11 @@ void prepare(hd_batch *b, int variant, int early_exit) {
     d.out_bytes = (uint32_t)J.out_bytes;
     d.max_dist = J.max_dist;
     d.check_every = check_interval(early_exit != 0, J.max_dist, J.g.W);
-    const uint64_t nb = (J.n + TI - 1) / TI;
-    tiles += tiles_before(nb, M);
+    tiles += tiles_before((J.n + TI - 1) / TI, TI / TJ);
     b->host_jobs.push_back(d);
   }
+  b->only4 = true;
+  for (const auto &d : b->host_jobs) b->only4 = b->only4 && d.P == 4;
   b->total_tiles = tiles;
   b->enc_blocks = blocks;
   b->n_jobs_active = (uint32_t)b->host_jobs.size();
@@ -257,8 +266,9 @@ void prepare(hd_batch *b, int variant, int early_exit) {
   int dev = 0, cus = 0, per_cu = 0;
   HD_HIP(hipGetDevice(&dev));
   HD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  HD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_of(b->variant),
-                                                      kWave * v.NW, 0));
+  HD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
+                                                      batch_kernel_of(b->variant, b->only4),
+                                                      kWave * kBatchWaves, 0));
   b->persistent_blocks = (uint32_t)std::max<uint64_t>(
       1, std::min<uint64_t>((uint64_t)cus * std::max(per_cu, 1), tiles));
 
@@ -399,9 +409,9 @@ int hd_batch_variant_count(void) { return kNumBatchVariants; }
 int hd_batch_variant_info(int variant, int *rows_per_lane, int *cols_per_wave, int *waves) {
   return guarded([&] {
     HD_REQUIRE(variant >= 0 && variant < kNumBatchVariants, "unknown batch variant");
-    if (rows_per_lane) *rows_per_lane = kBatchVariants[variant].R;
-    if (cols_per_wave) *cols_per_wave = kBatchVariants[variant].C;
-    if (waves) *waves = kBatchVariants[variant].NW;
+    if (rows_per_lane) *rows_per_lane = kShapes[variant].R;
+    if (cols_per_wave) *cols_per_wave = kShapes[variant].C;
+    if (waves) *waves = kBatchWaves;
   });
 }
 

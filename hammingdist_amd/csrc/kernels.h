@@ -50,12 +50,14 @@ __device__ __forceinline__ void tile_coords(uint64_t t, uint32_t M, uint32_t &b,
 }
 
 // One wave's share of a tile: rows i0 + 64 q + lane (q < R) against columns
-// j0 .. j0 + C - 1, stored into the triangle at cell i(i-1)/2 + j - cell_offset.
-template <int P, int R, int C, typename OutT>
+// j0 .. j0 + C - 1, stored into the triangle at cell i(i-1)/2 + j - cell_offset
+// as out_bytes-wide cells (1, 2 or 4; a constant where the caller's is).
+template <int P, int R, int C>
 __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, uint32_t n,
                                           uint32_t W, uint32_t i0, uint32_t j0,
-                                          OutT *__restrict__ out, uint64_t cell_offset,
-                                          uint32_t max_dist, uint32_t check_every) {
+                                          void *__restrict__ out, uint32_t out_bytes,
+                                          uint64_t cell_offset, uint32_t max_dist,
+                                          uint32_t check_every) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t i_end = min(i0 + (uint32_t)(kWave * R), n);
   if (j0 + 1 >= i_end) return; // no pair j < i inside this wave's block
@@ -93,10 +95,14 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
       for (int q = 0; q < R; ++q)
 #pragma unroll
         for (int c = 0; c < C; ++c) {
+          // s = a0&b0; s = (ap&bp)|s ...; s = ~((a_last&b_last)|s); one
+          // v_bitop3 per plane after the first (truth tables over
+          // src0=0xF0, src1=0xCC, src2=0xAA: 0xEA = (x&y)|z, 0x15 = its NOT).
           uint32_t s = a[q][0] & bw[c][0];
 #pragma unroll
-          for (int p = 1; p < P - 1; ++p) s |= a[q][p] & bw[c][p];
-          s = ~(s | (a[q][P - 1] & bw[c][P - 1]));
+          for (int p = 1; p < P - 1; ++p)
+            s = __builtin_amdgcn_bitop3_b32(a[q][p], bw[c][p], s, 0xEA);
+          s = __builtin_amdgcn_bitop3_b32(a[q][P - 1], bw[c][P - 1], s, 0x15);
           acc[q][c] += __builtin_popcount(s);
         }
       pa += P * kWave;
@@ -116,10 +122,16 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   for (int q = 0; q < R; ++q) {
     const uint32_t i = i0 + kWave * q + lane;
     if (i < n) {
-      OutT *row = out + ((uint64_t)i * (i - 1) / 2 + j0 - cell_offset);
+      const uint64_t x = (uint64_t)i * (i - 1) / 2 + j0 - cell_offset;
 #pragma unroll
-      for (int c = 0; c < C; ++c)
-        if (j0 + c < i) row[c] = (OutT)min(acc[q][c], max_dist);
+      for (int c = 0; c < C; ++c) {
+        if (j0 + c < i) {
+          const uint32_t v = min(acc[q][c], max_dist);
+          if (out_bytes == 1) static_cast<uint8_t *>(out)[x + c] = (uint8_t)v;
+          else if (out_bytes == 2) static_cast<uint16_t *>(out)[x + c] = (uint16_t)v;
+          else static_cast<uint32_t *>(out)[x + c] = v;
+        }
+      }
     }
   }
 }
@@ -142,8 +154,8 @@ __global__ __launch_bounds__(kWave *NW) void k_allpairs(
   uint32_t b, cb;
   tile_coords(t, TI / TJ, b, cb);
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  tile_wave<P, R, C, OutT>(planes, n, W, b * TI, cb * TJ + wave * C, out, cell_offset,
-                           max_dist, check_every);
+  tile_wave<P, R, C>(planes, n, W, b * TI, cb * TJ + wave * C, out, sizeof(OutT), cell_offset,
+                     max_dist, check_every);
 }
 
 // ---------------------------------------------------------------------------

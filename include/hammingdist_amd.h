@@ -177,8 +177,8 @@ int hd_batch_destroy(hd_batch *batch);
 int hd_batch_add(hd_batch *batch, const uint8_t *packed, uint64_t n,
                  uint64_t bytes_per_seq, int semantics, int out_bytes,
                  uint32_t max_dist, uint64_t *job);
-/* Builds the tile list with tile shape `variant` (see hd_batch_variant_info;
- * -1 = default) and captures the graph.                                     */
+/* Builds the tile list with batch variant `variant` (-1 = default; see
+ * hd_batch_variant_info) and captures the graph.                            */
 int hd_batch_prepare(hd_batch *batch, int variant, int early_exit);
 /* Replays the captured graph on `stream` (asynchronous).                    */
 int hd_batch_run(hd_batch *batch, void *stream);
@@ -191,6 +191,8 @@ int hd_batch_output_dev(hd_batch *batch, uint64_t job, void **d_cells,
                         uint64_t *cells);
 int hd_batch_info(hd_batch *batch, uint64_t *jobs, uint64_t *tiles,
                   uint32_t *blocks, int *variant);
+/* A batch variant is the tile shape every job of the launch uses: 64 R rows
+ * by W C columns (R rows per lane, C columns per wave, W waves).            */
 int hd_batch_variant_count(void);
 int hd_batch_variant_info(int variant, int *rows_per_lane, int *cols_per_wave,
                           int *waves);
