@@ -42,6 +42,8 @@ struct EncodeJobDev {
 };
 
 // Index of the job whose range [begin[k], begin[k+1]) holds x (begin sorted).
+// x is wave-uniform; the result is made explicitly uniform so that the job's
+// fields are scalar loads and everything derived from them stays in SGPRs.
 template <typename Job>
 __device__ __forceinline__ uint32_t find_job(const Job *__restrict__ jobs, uint32_t n_jobs,
                                              uint64_t x, uint64_t Job::*begin) {
@@ -51,7 +53,13 @@ __device__ __forceinline__ uint32_t find_job(const Job *__restrict__ jobs, uint3
     if (jobs[mid].*begin <= x) lo = mid;
     else hi = mid;
   }
-  return lo;
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
 }
 
 __global__ __launch_bounds__(256) void k_batch_encode(const EncodeJobDev *__restrict__ jobs,
@@ -87,7 +95,7 @@ __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_allpairs(
   for (;;) {
     if (threadIdx.x == 0) s_t = atomicAdd(counter, 1ull);
     __syncthreads();
-    const uint64_t t = s_t;
+    const uint64_t t = uniform64(s_t);
     __syncthreads();
     if (t >= total_tiles) break;
     const uint32_t j = find_job(jobs, n_jobs, t, &BatchJobDev::tile_begin);
@@ -120,6 +128,50 @@ __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_allpairs(
   }
 }
 
+// The same tiles as one grid: workgroup g computes tile g of the batch's list
+// (the hardware dispatcher balances the load; the list is in LPT order).
+template <int R, int C, bool ONLY4>
+__global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_grid(
+    const BatchJobDev *__restrict__ jobs, uint32_t n_jobs, uint64_t tile_begin) {
+  constexpr uint32_t TI = kWave * R;
+  constexpr uint32_t TJ = kBatchWaves * C;
+  const uint64_t t = tile_begin + blockIdx.x;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint32_t j = find_job(jobs, n_jobs, t, &BatchJobDev::tile_begin);
+  const BatchJobDev &J = jobs[j];
+  uint32_t b, cb;
+  tile_coords(t - J.tile_begin, TI / TJ, b, cb);
+  const uint32_t i0 = b * TI, j0 = cb * TJ + wave * C;
+  constexpr int CW = C > 8 ? 8 : C;
+  if (ONLY4 || J.P == 4) {
+    tile_wave<4, R, C>(J.planes, J.n, J.W, i0, j0, J.out, J.out_bytes, 0, J.max_dist,
+                       J.check_every);
+  } else if (J.P == 5) {
+    for (uint32_t h = 0; h < (uint32_t)(C / CW); ++h)
+      tile_wave<5, R, CW>(J.planes, J.n, J.W, i0, j0 + h * CW, J.out, J.out_bytes, 0,
+                          J.max_dist, J.check_every);
+  } else {
+    for (uint32_t h = 0; h < (uint32_t)(C / CW); ++h)
+      tile_wave<6, R, CW>(J.planes, J.n, J.W, i0, j0 + h * CW, J.out, J.out_bytes, 0,
+                          J.max_dist, J.check_every);
+  }
+}
+
+template <int V> static const void *grid_kernel_v(bool only4) {
+  return only4 ? reinterpret_cast<const void *>(&k_batch_grid<kShapes[V].R, kShapes[V].C, true>)
+               : reinterpret_cast<const void *>(&k_batch_grid<kShapes[V].R, kShapes[V].C, false>);
+}
+
+static const void *grid_kernel_of(int v, bool only4) {
+  switch (v) {
+  case 0: return grid_kernel_v<0>(only4);
+  case 1: return grid_kernel_v<1>(only4);
+  case 2: return grid_kernel_v<2>(only4);
+  case 3: return grid_kernel_v<3>(only4);
+  default: return grid_kernel_v<4>(only4);
+  }
+}
+
 template <int V> static const void *batch_kernel_v(bool only4) {
   return only4 ? reinterpret_cast<const void *>(
                      &k_batch_allpairs<kShapes[V].R, kShapes[V].C, true>)
@@ -137,7 +189,9 @@ static const void *batch_kernel_of(int v, bool only4) {
   }
 }
 
-static constexpr int kNumBatchVariants = kNumShapes;
+// Batch variant v: shape v % kNumShapes; persistent workgroups with an
+// atomic tile queue for v < kNumShapes, one workgroup per tile above.
+static constexpr int kNumBatchVariants = 2 * kNumShapes;
 
 } // namespace hd
 
@@ -185,6 +239,19 @@ void enqueue_batch(hd_batch *b, hipStream_t s) {
     HD_HIP(hipGetLastError());
   }
   if (!b->total_tiles) return;
+  const int shape = b->variant % kNumShapes;
+  if (b->variant >= kNumShapes) {
+    const BatchJobDev *jp = b->d_jobs.get<BatchJobDev>();
+    uint32_t nj = b->n_jobs_active;
+    for (uint64_t t0 = 0; t0 < b->total_tiles; t0 += 0x7fffffffull) {
+      const uint64_t cnt = std::min<uint64_t>(0x7fffffffull, b->total_tiles - t0);
+      uint64_t tb = t0;
+      void *gargs[] = {(void *)&jp, (void *)&nj, (void *)&tb};
+      HD_HIP(hipLaunchKernel(grid_kernel_of(shape, b->only4), dim3((uint32_t)cnt),
+                             dim3(kWave * kBatchWaves), gargs, 0, s));
+    }
+    return;
+  }
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s,
                      b->d_counter.get<unsigned long long>());
   HD_HIP(hipGetLastError());
@@ -193,7 +260,7 @@ void enqueue_batch(hd_batch *b, hipStream_t s) {
   uint64_t tt = b->total_tiles;
   unsigned long long *ctr = b->d_counter.get<unsigned long long>();
   void *args[] = {(void *)&jp, (void *)&nj, (void *)&tt, (void *)&ctr};
-  HD_HIP(hipLaunchKernel(batch_kernel_of(b->variant, b->only4), dim3(b->persistent_blocks),
+  HD_HIP(hipLaunchKernel(batch_kernel_of(shape, b->only4), dim3(b->persistent_blocks),
                          dim3(kWave * kBatchWaves), args, 0, s));
 }
 
@@ -204,7 +271,7 @@ void prepare(hd_batch *b, int variant, int early_exit) {
   b->variant = variant < 0 ? 0 : variant;
   b->early_exit = early_exit;
   hipStream_t s = lib_stream();
-  const Shape sh = kShapes[b->variant];
+  const Shape sh = kShapes[b->variant % kNumShapes];
   const uint64_t TI = 64ull * sh.R, TJ = (uint64_t)kBatchWaves * sh.C;
   auto tile_work = [&](const hd_batch::Job &J) { return (uint64_t)J.g.W * J.g.code.planes; };
 
@@ -267,7 +334,8 @@ void prepare(hd_batch *b, int variant, int early_exit) {
   HD_HIP(hipGetDevice(&dev));
   HD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   HD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
-                                                      batch_kernel_of(b->variant, b->only4),
+                                                      batch_kernel_of(b->variant % kNumShapes,
+                                                                      b->only4),
                                                       kWave * kBatchWaves, 0));
   b->persistent_blocks = (uint32_t)std::max<uint64_t>(
       1, std::min<uint64_t>((uint64_t)cus * std::max(per_cu, 1), tiles));
@@ -409,8 +477,8 @@ int hd_batch_variant_count(void) { return kNumBatchVariants; }
 int hd_batch_variant_info(int variant, int *rows_per_lane, int *cols_per_wave, int *waves) {
   return guarded([&] {
     HD_REQUIRE(variant >= 0 && variant < kNumBatchVariants, "unknown batch variant");
-    if (rows_per_lane) *rows_per_lane = kShapes[variant].R;
-    if (cols_per_wave) *cols_per_wave = kShapes[variant].C;
+    if (rows_per_lane) *rows_per_lane = kShapes[variant % kNumShapes].R;
+    if (cols_per_wave) *cols_per_wave = kShapes[variant % kNumShapes].C;
     if (waves) *waves = kBatchWaves;
   });
 }

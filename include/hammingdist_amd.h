@@ -191,8 +191,10 @@ int hd_batch_output_dev(hd_batch *batch, uint64_t job, void **d_cells,
                         uint64_t *cells);
 int hd_batch_info(hd_batch *batch, uint64_t *jobs, uint64_t *tiles,
                   uint32_t *blocks, int *variant);
-/* A batch variant is the tile shape every job of the launch uses: 64 R rows
- * by W C columns (R rows per lane, C columns per wave, W waves).            */
+/* A batch variant is the tile shape every job of the launch uses, 64 R rows
+ * by W C columns (R rows per lane, C columns per wave, W waves), and the
+ * schedule: variants below hd_batch_variant_count()/2 run persistent
+ * workgroups on an atomic tile queue, the others one workgroup per tile.    */
 int hd_batch_variant_count(void);
 int hd_batch_variant_info(int variant, int *rows_per_lane, int *cols_per_wave,
                           int *waves);
