@@ -37,7 +37,7 @@ struct EncodeJobDev {
   uint32_t *planes;
   uint64_t bpr;
   uint64_t block_begin; // first encode block of this job
-  uint32_t n, n_pad, W, P, runs, pad;
+  uint32_t n, n_pad, W, P, runs, identity;
   Table16 tbl;
 };
 
@@ -73,7 +73,8 @@ __global__ __launch_bounds__(256) void k_batch_encode(const EncodeJobDev *__rest
   const uint32_t row_blocks = J.n_pad / 256;
   const uint32_t i = (uint32_t)(local % row_blocks) * 256 + threadIdx.x;
   const uint32_t run = (uint32_t)(local / row_blocks);
-  encode_run(J.packed, J.n, J.bpr, J.W, (int)J.P, s_tbl, J.planes, i, run * kEncodeRun);
+  encode_run(J.packed, J.n, J.bpr, J.W, (int)J.P, s_tbl, J.planes, i, run * kEncodeRun,
+             J.identity != 0);
 }
 
 __global__ void k_reset_counter(unsigned long long *counter) { *counter = 0; }
@@ -298,6 +299,7 @@ void prepare(hd_batch *b, int variant, int early_exit) {
     e.P = (uint32_t)J.g.code.planes;
     e.runs = (J.g.W + kEncodeRun - 1) / kEncodeRun;
     std::memcpy(e.tbl.v, J.g.code.table, 16);
+    e.identity = is_identity(J.g.code) ? 1u : 0u;
     blocks += (uint64_t)(e.n_pad / 256) * e.runs;
     enc.push_back(e);
     if (J.n < 2) continue;

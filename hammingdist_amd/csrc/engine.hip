@@ -233,14 +233,21 @@ static void check_max_dist(int out_bytes, uint32_t max_dist) {
 __global__ __launch_bounds__(256) void k_encode(const uint8_t *__restrict__ packed,
                                                 uint32_t n, uint64_t bpr,
                                                 uint32_t n_pad, uint32_t W,
-                                                int P, Table16 tbl,
+                                                int P, Table16 tbl, int identity,
                                                 uint32_t *__restrict__ planes) {
   __shared__ uint8_t s_tbl[16];
   if (threadIdx.x < 16) s_tbl[threadIdx.x] = tbl.v[threadIdx.x];
   __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_pad) return;
-  encode_run(packed, n, bpr, W, P, s_tbl, planes, i, blockIdx.y * kEncodeRun);
+  encode_run(packed, n, bpr, W, P, s_tbl, planes, i, blockIdx.y * kEncodeRun, identity != 0);
+}
+
+bool is_identity(const PlaneCode &c) {
+  if (c.planes != 4) return false;
+  for (int v = 0; v < 16; ++v)
+    if (c.table[v] != v) return false;
+  return true;
 }
 
 void launch_encode(const Geometry &g, const uint8_t *d_packed, uint32_t *d_planes,
@@ -251,7 +258,8 @@ void launch_encode(const Geometry &g, const uint8_t *d_packed, uint32_t *d_plane
   HD_REQUIRE(runs <= 65535, "sequences too long for the encode grid");
   dim3 grid((uint32_t)((g.n_pad + 255) / 256), runs);
   hipLaunchKernelGGL(k_encode, grid, dim3(256), 0, s, d_packed, (uint32_t)g.n,
-                     g.bpr, (uint32_t)g.n_pad, g.W, g.code.planes, t, d_planes);
+                     g.bpr, (uint32_t)g.n_pad, g.W, g.code.planes, t,
+                     is_identity(g.code) ? 1 : 0, d_planes);
   HD_HIP(hipGetLastError());
 }
 

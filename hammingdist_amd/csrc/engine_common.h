@@ -140,6 +140,8 @@ struct PlaneCode {
 };
 
 PlaneCode plane_code(int semantics, uint32_t nibble_mask);
+// True for the 4-plane code equal to the nibble (the dense semantics).
+bool is_identity(const PlaneCode &c);
 
 inline uint64_t tri_cells(uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; }
 

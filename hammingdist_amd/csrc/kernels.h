@@ -75,8 +75,12 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
 
   // Row group g starts at word g * W * P * 64; one step of it is P * 64 words.
   const size_t group = (size_t)W * P * kWave;
-  const uint32_t *pa = planes + (size_t)(i0 / kWave) * group + lane;
-  const uint32_t *pb = planes + (size_t)(j0 / kWave) * group + (j0 % kWave);
+  // The planes are read-only for the whole kernel: reading them through the
+  // constant address space lets the wave-uniform column words be scalar
+  // loads even where the pointer itself came from memory.
+  using cptr = const __attribute__((address_space(4))) uint32_t *;
+  cptr pa = (cptr)(planes + (size_t)(i0 / kWave) * group + lane);
+  cptr pb = (cptr)(planes + (size_t)(j0 / kWave) * group + (j0 % kWave));
   uint32_t k = 0;
   while (k < W) {
     const uint32_t k_end = min(W, k + check_every);
@@ -167,12 +171,49 @@ __global__ __launch_bounds__(kWave *NW) void k_allpairs(
 // encode as '-' (table entry 15, all planes set), which never counts.
 constexpr int kEncodeRun = 8;
 
+// Bit p of each of the 8 nibbles of v, gathered into bits 0..7 (nibble k ->
+// bit k): mask the bits, then halve the spacing three times.
+__device__ __forceinline__ uint32_t nibble_bits(uint32_t v, int p) {
+  uint32_t x = (v >> p) & 0x11111111u;
+  x = (x | (x >> 3)) & 0x03030303u;
+  x = (x | (x >> 6)) & 0x000F000Fu;
+  return (x | (x >> 12)) & 0xFFu;
+}
+
 __device__ __forceinline__ void encode_run(const uint8_t *__restrict__ packed, uint32_t n,
                                            uint64_t bpr, uint32_t W, int P,
                                            const uint8_t *s_tbl,
                                            uint32_t *__restrict__ planes, uint32_t i,
-                                           uint32_t k0) {
+                                           uint32_t k0, bool identity = false) {
   const uint8_t *row = packed + (uint64_t)i * bpr;
+  if (identity) {
+    // The 4-plane dense code is the nibble itself: plane p is bit p of every
+    // nibble, with no table lookup.
+    for (int kk = 0; kk < kEncodeRun; ++kk) {
+      const uint32_t k = k0 + kk;
+      if (k >= W) break;
+      const uint64_t off = (uint64_t)k * 16;
+      uint32_t bytes[4];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint64_t o = off + 8 * h;
+        if (i < n && o + 8 <= bpr) {
+          const uint2 u = *reinterpret_cast<const uint2 *>(row + o);
+          bytes[2 * h] = u.x;
+          bytes[2 * h + 1] = u.y;
+        } else {
+          bytes[2 * h] = 0xFFFFFFFFu;
+          bytes[2 * h + 1] = 0xFFFFFFFFu;
+        }
+      }
+      uint32_t *dst = planes + (((uint64_t)(i / 64) * W + k) * 4) * 64 + (i % 64);
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        dst[p * 64] = nibble_bits(bytes[0], p) | (nibble_bits(bytes[1], p) << 8) |
+                      (nibble_bits(bytes[2], p) << 16) | (nibble_bits(bytes[3], p) << 24);
+    }
+    return;
+  }
   for (int kk = 0; kk < kEncodeRun; ++kk) {
     const uint32_t k = k0 + kk;
     if (k >= W) break;
