@@ -364,6 +364,55 @@ def run_batched(args, torch, capi, workloads, np):
     return res
 
 
+def run_serving(args, torch, capi, workloads, np):
+    from hammingdist_amd import serving
+
+    spec = workloads.serving_db(n=args.serve_rows)
+    t = time.perf_counter()
+    db = workloads.generate_packed(spec)
+    log(f"serving: generated db {db.shape} ({db.nbytes / 1e9:.1f} GB) in "
+        f"{time.perf_counter() - t:.1f}s")
+    t = time.perf_counter()
+    srv = capi.Server(db, device_bytes=24 << 30, host_bytes=4 << 30, streams=3, max_batch=32)
+    log(f"serving: db resident in HBM after {time.perf_counter() - t:.1f}s")
+    reqs = serving.make_requests(db, spec.L, args.serve_requests, seed=1004)
+    # bit-exact spot check: a few requests alone against the oracle
+    O = oracle()
+    for r in reqs[:8]:
+        got = serving.run_alone(srv, [r])[0]
+        if r.kind == "query":
+            idx = np.linspace(0, r.rows.size - 1, 50).astype(np.int64)
+            want = [O.distance_dense(r.query, db[int(r.rows[k])]) for k in idx]
+            assert got[idx].tolist() == want
+        else:
+            sub = np.ascontiguousarray(db[r.rows[:200].astype(np.int64)])
+            want = O.all_pairs_dense(sub, 1, 255)
+            assert np.array_equal(got[: want.size], want)
+    # capacity: everything submitted at once
+    warm = reqs[: max(8, len(reqs) // 10)]
+    serving.closed_loop(srv, warm)
+    _, secs, lat_c = serving.closed_loop(srv, reqs)
+    capacity = len(reqs) / secs
+    # open loop at 70% of capacity
+    rate = 0.7 * capacity
+    _, secs_o, lat_o = serving.open_loop(srv, reqs, rate, seed=7)
+    st = srv.stats()
+    srv.close()
+    n_q = sum(r.kind == "query" for r in reqs)
+    rows_scanned = sum(r.rows.size for r in reqs if r.kind == "query")
+    res = {"db_rows": int(db.shape[0]), "db_GB": db.nbytes / 1e9, "requests": len(reqs),
+           "queries": n_q, "all_pairs_jobs": len(reqs) - n_q,
+           "capacity_requests_per_s": capacity,
+           "closed_loop": serving.percentiles(lat_c),
+           "offered_rate": rate, "achieved_requests_per_s": len(reqs) / secs_o,
+           "open_loop": serving.percentiles(lat_o),
+           "query_rows_GB_per_s_closed": rows_scanned * db.shape[1] / secs / 1e9,
+           "stats": st}
+    log(f"serving: capacity {capacity:.0f} req/s; open loop at {rate:.0f}/s "
+        f"p50 {res['open_loop']['p50_ms']:.2f} ms p99 {res['open_loop']['p99_ms']:.2f} ms")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -375,17 +424,20 @@ def main() -> int:
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--serve-rows", type=int, default=1_000_000)
+    ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-early-exit-run", action="store_true")
     ap.add_argument("--no-refdist", action="store_true")
     ap.add_argument("--no-wide", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline step (for profiling)")
     args = ap.parse_args()
     if args.headline_only:
         args.no_cpu_baseline = args.no_early_exit_run = args.no_refdist = True
-        args.no_wide = args.no_batched = True
+        args.no_wide = args.no_batched = args.no_serving = True
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -421,6 +473,8 @@ def main() -> int:
         result["wide"] = run_wide(args, torch, capi, workloads, np)
     if not args.no_batched:
         result["batched"] = run_batched(args, torch, capi, workloads, np)
+    if not args.no_serving:
+        result["serving"] = run_serving(args, torch, capi, workloads, np)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(packed, args.L)

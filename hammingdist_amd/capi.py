@@ -25,6 +25,12 @@ HD_SEM_SPARSE = 1
 HD_SEM_SPARSE_X = 2
 HD_SEM_CHAR_X = 3
 
+HD_REQ_PENDING = 0
+HD_REQ_ADMITTED = 1
+HD_REQ_RUNNING = 2
+HD_REQ_DONE = 3
+HD_REQ_FAILED = 4
+
 u8p = C.POINTER(C.c_uint8)
 u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
@@ -67,6 +73,16 @@ SIGNATURES = {
     "hd_batch_info": (C.c_int, [vp, u64p, u64p, u32p, intp]),
     "hd_batch_variant_count": (C.c_int, []),
     "hd_batch_variant_info": (C.c_int, [C.c_int, intp, intp, intp]),
+    "hd_server_create": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int,
+                                   C.c_uint64, C.POINTER(vp)]),
+    "hd_server_destroy": (C.c_int, [vp]),
+    "hd_server_submit_query": (C.c_int, [vp, vp, vp, C.c_uint64, C.c_int, u64p]),
+    "hd_server_submit_all_pairs": (C.c_int, [vp, vp, C.c_uint64, C.c_int, C.c_int, C.c_uint32,
+                                             u64p]),
+    "hd_server_wait": (C.c_int, [vp, C.c_uint64, C.c_double, intp]),
+    "hd_server_result": (C.c_int, [vp, C.c_uint64, vp, C.c_uint64, C.POINTER(C.c_double)]),
+    "hd_server_release": (C.c_int, [vp, C.c_uint64]),
+    "hd_server_stats": (C.c_int, [vp, u64p, u64p, u64p, u64p, u64p]),
     "hd_variant_count": (C.c_int, []),
     "hd_variant_info": (C.c_int, [C.c_int, intp, intp, intp, intp]),
 }
@@ -276,3 +292,80 @@ def batch_variants() -> list[dict]:
         out.append({"variant": v, "rows_per_lane": r.value, "cols_per_wave": c.value,
                     "waves": w.value})
     return out
+
+
+class Server:
+    """hd_server_*: a device-resident packed database serving reference
+    distance queries and small all-pairs jobs with continuous batching."""
+
+    def __init__(self, db, device_bytes: int = 8 << 30, host_bytes: int = 2 << 30,
+                 streams: int = 2, max_batch: int = 64):
+        import numpy as np
+
+        d = np.ascontiguousarray(db)
+        self.n_db, self.bps = d.shape
+        self._h = C.c_void_p()
+        check(lib().hd_server_create(d.ctypes.data, self.n_db, self.bps, device_bytes,
+                                     host_bytes, streams, max_batch, C.byref(self._h)))
+        self._meta = {}
+
+    def close(self):
+        if self._h:
+            lib().hd_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit_query(self, query, rows, semantics: int = HD_SEM_DENSE) -> int:
+        import numpy as np
+
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        r = np.ascontiguousarray(rows, dtype=np.uint64)
+        rid = C.c_uint64()
+        check(lib().hd_server_submit_query(self._h, q.ctypes.data, r.ctypes.data, r.size,
+                                           semantics, C.byref(rid)))
+        self._meta[rid.value] = ("query", r.size, 4)
+        return rid.value
+
+    def submit_all_pairs(self, rows, semantics: int = HD_SEM_DENSE, out_bytes: int = 1,
+                         max_dist: int = 255) -> int:
+        import numpy as np
+
+        r = np.ascontiguousarray(rows, dtype=np.uint64)
+        rid = C.c_uint64()
+        check(lib().hd_server_submit_all_pairs(self._h, r.ctypes.data, r.size, semantics,
+                                               out_bytes, max_dist, C.byref(rid)))
+        self._meta[rid.value] = ("all_pairs", r.size, out_bytes)
+        return rid.value
+
+    def wait(self, rid: int, timeout: float = -1.0) -> int:
+        st = C.c_int()
+        check(lib().hd_server_wait(self._h, rid, timeout, C.byref(st)))
+        return st.value
+
+    def result(self, rid: int, release: bool = True):
+        """(result array, latency seconds)."""
+        import numpy as np
+
+        kind, m, ob = self._meta[rid]
+        if kind == "query":
+            out = np.zeros(m, np.uint32)
+        else:
+            out = np.zeros(m * (m - 1) // 2, {1: np.uint8, 2: np.uint16, 4: np.uint32}[ob])
+        lat = C.c_double()
+        check(lib().hd_server_result(self._h, rid, out.ctypes.data if out.size else None,
+                                     out.nbytes, C.byref(lat)))
+        if release:
+            check(lib().hd_server_release(self._h, rid))
+            del self._meta[rid]
+        return out, lat.value
+
+    def stats(self) -> dict:
+        v = [C.c_uint64() for _ in range(5)]
+        check(lib().hd_server_stats(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(["done", "batches", "overlapped_batches", "max_running",
+                         "device_used"], [x.value for x in v]))

@@ -1,0 +1,599 @@
+// Serving loop: a packed sequence database resident in HBM, requests
+// (reference-distance queries against row subsets, small all-pairs jobs on
+// database rows) admitted by device-memory footprint and launched in batches
+// as they arrive, with several batches in flight on separate streams.
+// C ABI: hd_server_* in include/hammingdist_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine_common.h"
+#include "kernels.h"
+
+namespace hd {
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+struct QueryDev {
+  const uint2 *query;
+  const uint64_t *rows;
+  uint32_t *out;
+  uint64_t m;
+  uint64_t block_begin; // first block of this query in the launch
+  int semantics;
+  int pad;
+};
+
+constexpr uint32_t kRowsPerBlock = 64; // 4 wavefronts x 16 rows
+
+template <int SEM>
+__device__ __forceinline__ uint32_t count_sites(uint32_t q, uint32_t r) {
+  if constexpr (SEM == HD_SEM_DENSE) {
+    // (q & r) nibble == 0, nibble bits smeared into bit 3
+    uint32_t c = q & r;
+    c |= c << 1;
+    c |= c << 2;
+    return __builtin_popcount(~c & 0x88888888u);
+  } else {
+    // (q & r) == 0 || (q != r && neither is '-')
+    uint32_t z = q & r;
+    z |= z << 1;
+    z |= z << 2;
+    uint32_t d = q ^ r;
+    d |= d << 1;
+    d |= d << 2;
+    uint32_t gq = q & (q << 1);
+    gq &= gq << 2;
+    uint32_t gr = r & (r << 1);
+    gr &= gr << 2;
+    return __builtin_popcount((~z | (d & ~(gq | gr))) & 0x88888888u);
+  }
+}
+
+template <int SEM>
+__device__ __forceinline__ uint32_t row_distance(const uint2 *__restrict__ q,
+                                                 const uint2 *__restrict__ row, uint64_t units,
+                                                 uint32_t lane) {
+  uint32_t cnt = 0;
+  for (uint64_t u = lane; u < units; u += kWave) {
+    const uint2 a = row[u];
+    const uint2 b = q[u];
+    cnt += count_sites<SEM>(b.x, a.x) + count_sites<SEM>(b.y, a.y);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+  return cnt;
+}
+
+// Many queries in one launch: block g belongs to the query whose block range
+// holds g and scans 64 of its rows, one wavefront per row at a time.
+__global__ __launch_bounds__(256) void k_query_batch(const QueryDev *__restrict__ qs,
+                                                     uint32_t n_q,
+                                                     const uint8_t *__restrict__ db,
+                                                     uint64_t bpr) {
+  uint32_t lo = 0, hi = n_q;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (qs[mid].block_begin <= blockIdx.x) lo = mid;
+    else hi = mid;
+  }
+  const QueryDev &Q = qs[__builtin_amdgcn_readfirstlane(lo)];
+  const uint64_t r0 = (blockIdx.x - Q.block_begin) * kRowsPerBlock;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t units = bpr / 8;
+  for (uint64_t k = r0 + wave; k < r0 + kRowsPerBlock && k < Q.m; k += 4) {
+    const uint2 *row = reinterpret_cast<const uint2 *>(db + Q.rows[k] * bpr);
+    const uint32_t d = Q.semantics == HD_SEM_DENSE
+                           ? row_distance<HD_SEM_DENSE>(Q.query, row, units, lane)
+                           : row_distance<HD_SEM_CHAR_X>(Q.query, row, units, lane);
+    if (lane == 0) Q.out[k] = d;
+  }
+}
+
+// Copies database rows rows[0..n) into a contiguous packed buffer (one block
+// per row, 8-byte moves).
+__global__ __launch_bounds__(256) void k_gather_rows(const uint8_t *__restrict__ db, uint64_t bpr,
+                                                     const uint64_t *__restrict__ rows,
+                                                     uint8_t *__restrict__ out) {
+  const uint64_t r = rows[blockIdx.x];
+  const uint2 *src = reinterpret_cast<const uint2 *>(db + r * bpr);
+  uint2 *dst = reinterpret_cast<uint2 *>(out + (uint64_t)blockIdx.x * bpr);
+  for (uint64_t u = threadIdx.x; u < bpr / 8; u += blockDim.x) dst[u] = src[u];
+}
+
+// ---------------------------------------------------------------------------
+// first-fit arena over one allocation (device or pinned host)
+// ---------------------------------------------------------------------------
+class Arena {
+public:
+  void init(uint8_t *base, uint64_t size) {
+    base_ = base;
+    size_ = size;
+    free_.clear();
+    sizes_.clear();
+    if (size) free_[0] = size;
+    used_ = 0;
+  }
+  static uint64_t round(uint64_t b) { return std::max<uint64_t>(256, (b + 255) / 256 * 256); }
+  // Offset of a block of `bytes` or UINT64_MAX when no free run is that large.
+  uint64_t alloc(uint64_t bytes) {
+    bytes = round(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        const uint64_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > bytes) free_[off + bytes] = len - bytes;
+        used_ += bytes;
+        sizes_[off] = bytes;
+        return off;
+      }
+    }
+    return UINT64_MAX;
+  }
+  void release(uint64_t off) {
+    auto s = sizes_.find(off);
+    if (s == sizes_.end()) return;
+    uint64_t len = s->second;
+    sizes_.erase(s);
+    used_ -= len;
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && off + len == next->first) {
+      len += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        prev->second += len;
+        return;
+      }
+    }
+    free_[off] = len;
+  }
+  uint8_t *at(uint64_t off) const { return base_ + off; }
+  uint64_t used() const { return used_; }
+  uint64_t size() const { return size_; }
+
+private:
+  uint8_t *base_ = nullptr;
+  uint64_t size_ = 0, used_ = 0;
+  std::map<uint64_t, uint64_t> free_;
+  std::unordered_map<uint64_t, uint64_t> sizes_;
+};
+
+// Device and pinned-host blocks of one request, as offsets inside them.
+//   query    device [query | rows | out]   host [query | rows | out]
+//   all-pairs device [rows | packed | planes | out]   host [rows | out]
+struct Layout {
+  uint64_t dev = 0, host = 0;
+  uint64_t d_query = 0, d_rows = 0, d_packed = 0, d_planes = 0, d_out = 0;
+  uint64_t h_rows = 0, h_out = 0, in_bytes = 0, out_bytes = 0;
+};
+
+} // namespace hd
+
+using namespace hd;
+
+struct hd_server {
+  enum Kind { QUERY = 0, ALLPAIRS = 1 };
+  struct Request {
+    uint64_t id = 0;
+    Kind kind = QUERY;
+    int state = 0; // HD_REQ_*
+    int semantics = HD_SEM_DENSE, out_bytes = 4;
+    uint32_t max_dist = 0;
+    uint64_t m = 0; // rows of a query, sequences of an all-pairs job
+    std::vector<uint64_t> rows;
+    std::vector<uint8_t> query;
+    Layout lay;
+    Geometry g;
+    uint64_t d_off = UINT64_MAX, h_off = UINT64_MAX;
+    std::chrono::steady_clock::time_point t_submit, t_done;
+  };
+  struct Batch {
+    int slot = 0;
+    std::vector<uint64_t> ids;
+    uint64_t table_d = UINT64_MAX, table_h = UINT64_MAX;
+  };
+
+  uint64_t n_db = 0, bpr = 0, max_batch = 64;
+  int n_slots = 2;
+  DeviceBuffer db, dev_mem;
+  PinnedBuffer host_mem;
+  Arena dev_arena, host_arena;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> events;
+  std::vector<bool> slot_busy;
+
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<uint64_t> pending;
+  std::deque<uint64_t> admitted;
+  std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+  std::vector<Batch> running;
+  uint64_t next_id = 1;
+  bool stopping = false;
+  std::string fatal;
+  std::thread worker;
+
+  uint64_t n_done = 0, n_batches = 0, n_overlapped = 0, max_running_reqs = 0,
+           batched_reqs = 0;
+
+  ~hd_server() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stopping = true;
+    }
+    cv_work.notify_all();
+    if (worker.joinable()) worker.join();
+    for (auto s : streams) (void)hipStreamSynchronize(s);
+    for (auto e : events) (void)hipEventDestroy(e);
+    for (auto s : streams) (void)hipStreamDestroy(s);
+  }
+
+  Layout layout(Request &r) const {
+    Layout L;
+    auto up = [](uint64_t b) { return Arena::round(b); };
+    if (r.kind == QUERY) {
+      L.d_query = 0;
+      L.d_rows = up(bpr);
+      L.d_out = L.d_rows + up(r.m * 8);
+      L.dev = L.d_out + up(r.m * 4);
+      L.h_rows = L.d_rows;
+      L.h_out = L.d_out;
+      L.host = L.dev;
+      L.in_bytes = L.d_out; // query and rows, staged together
+      L.out_bytes = r.m * 4;
+    } else {
+      L.d_rows = 0;
+      L.d_packed = up(r.m * 8);
+      L.d_planes = L.d_packed + up(r.m * bpr);
+      L.d_out = L.d_planes + up(r.g.total_words() * 4);
+      L.out_bytes = tri_cells(r.m) * r.out_bytes;
+      L.dev = L.d_out + up(L.out_bytes);
+      L.h_rows = 0;
+      L.h_out = up(r.m * 8);
+      L.host = L.h_out + up(L.out_bytes);
+      L.in_bytes = r.m * 8;
+    }
+    return L;
+  }
+
+  // Launches every request in ids on stream `slot`; returns the batch.
+  Batch launch(int slot, std::vector<uint64_t> ids) {
+    hipStream_t s = streams[slot];
+    Batch b;
+    b.slot = slot;
+    std::vector<QueryDev> qd;
+    uint64_t blocks = 0;
+    for (uint64_t id : ids) {
+      Request &r = *reqs[id];
+      uint8_t *dbase = dev_arena.at(r.d_off), *hbase = host_arena.at(r.h_off);
+      if (r.kind == QUERY) {
+        HD_HIP(hipMemcpyAsync(dbase, hbase, r.lay.in_bytes, hipMemcpyHostToDevice, s));
+        QueryDev d{};
+        d.query = reinterpret_cast<const uint2 *>(dbase + r.lay.d_query);
+        d.rows = reinterpret_cast<const uint64_t *>(dbase + r.lay.d_rows);
+        d.out = reinterpret_cast<uint32_t *>(dbase + r.lay.d_out);
+        d.m = r.m;
+        d.block_begin = blocks;
+        d.semantics = r.semantics;
+        blocks += (r.m + kRowsPerBlock - 1) / kRowsPerBlock;
+        qd.push_back(d);
+      } else {
+        HD_HIP(hipMemcpyAsync(dbase + r.lay.d_rows, hbase + r.lay.h_rows, r.lay.in_bytes,
+                              hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_gather_rows, dim3((uint32_t)r.m), dim3(256), 0, s,
+                           db.get<uint8_t>(), bpr,
+                           reinterpret_cast<const uint64_t *>(dbase + r.lay.d_rows),
+                           dbase + r.lay.d_packed);
+        HD_HIP(hipGetLastError());
+        launch_encode(r.g, dbase + r.lay.d_packed,
+                      reinterpret_cast<uint32_t *>(dbase + r.lay.d_planes), s);
+        if (r.m >= 2) {
+          const int v = default_variant(r.g.code.planes, r.m);
+          const uint64_t TI = 64ull * variant_of(v).R;
+          run_allpairs(r.g, reinterpret_cast<uint32_t *>(dbase + r.lay.d_planes), r.out_bytes,
+                       r.max_dist, true, v, 0, (r.m + TI - 1) / TI, dbase + r.lay.d_out, 0, s);
+        }
+      }
+    }
+    if (!qd.empty()) {
+      const uint64_t bytes = qd.size() * sizeof(QueryDev);
+      b.table_d = dev_arena.alloc(bytes);
+      b.table_h = host_arena.alloc(bytes);
+      if (b.table_d == UINT64_MAX || b.table_h == UINT64_MAX)
+        throw Error(HD_ERR_CAPACITY, "no arena space for the query table");
+      std::memcpy(host_arena.at(b.table_h), qd.data(), bytes);
+      HD_HIP(hipMemcpyAsync(dev_arena.at(b.table_d), host_arena.at(b.table_h), bytes,
+                            hipMemcpyHostToDevice, s));
+      HD_REQUIRE(blocks < (1ull << 31), "query batch too large");
+      hipLaunchKernelGGL(k_query_batch, dim3((uint32_t)blocks), dim3(256), 0, s,
+                         reinterpret_cast<const QueryDev *>(dev_arena.at(b.table_d)),
+                         (uint32_t)qd.size(), db.get<uint8_t>(), bpr);
+      HD_HIP(hipGetLastError());
+    }
+    for (uint64_t id : ids) {
+      Request &r = *reqs[id];
+      if (r.lay.out_bytes)
+        HD_HIP(hipMemcpyAsync(host_arena.at(r.h_off) + r.lay.h_out,
+                              dev_arena.at(r.d_off) + r.lay.d_out, r.lay.out_bytes,
+                              hipMemcpyDeviceToHost, s));
+      r.state = HD_REQ_RUNNING;
+    }
+    HD_HIP(hipEventRecord(events[slot], s));
+    b.ids = std::move(ids);
+    return b;
+  }
+
+  void complete(Batch &b, int state) {
+    const auto now = std::chrono::steady_clock::now();
+    for (uint64_t id : b.ids) {
+      Request &r = *reqs[id];
+      r.state = state;
+      r.t_done = now;
+      dev_arena.release(r.d_off);
+      r.d_off = UINT64_MAX;
+      ++n_done;
+    }
+    if (b.table_d != UINT64_MAX) dev_arena.release(b.table_d);
+    if (b.table_h != UINT64_MAX) host_arena.release(b.table_h);
+    slot_busy[b.slot] = false;
+  }
+
+  void loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      // 1. completed batches free their device memory
+      bool progressed = false;
+      for (size_t k = 0; k < running.size();) {
+        const hipError_t q = hipEventQuery(events[running[k].slot]);
+        if (q == hipErrorNotReady) {
+          ++k;
+          continue;
+        }
+        complete(running[k], q == hipSuccess ? HD_REQ_DONE : HD_REQ_FAILED);
+        running.erase(running.begin() + (long)k);
+        progressed = true;
+      }
+      if (progressed) cv_done.notify_all();
+      // 2. admission by footprint, in arrival order
+      while (!pending.empty()) {
+        Request &r = *reqs[pending.front()];
+        const uint64_t d = dev_arena.alloc(r.lay.dev);
+        if (d == UINT64_MAX) break;
+        const uint64_t h = host_arena.alloc(r.lay.host);
+        if (h == UINT64_MAX) {
+          dev_arena.release(d);
+          break;
+        }
+        r.d_off = d;
+        r.h_off = h;
+        uint8_t *hb = host_arena.at(h);
+        if (r.kind == QUERY) {
+          std::memcpy(hb, r.query.data(), bpr);
+          std::memcpy(hb + r.lay.h_rows, r.rows.data(), r.m * 8);
+        } else {
+          std::memcpy(hb + r.lay.h_rows, r.rows.data(), r.m * 8);
+        }
+        r.state = HD_REQ_ADMITTED;
+        admitted.push_back(r.id);
+        pending.pop_front();
+      }
+      // 3. everything admitted so far goes out as one batch on a free stream
+      if (!admitted.empty()) {
+        int slot = -1;
+        for (int s = 0; s < n_slots; ++s)
+          if (!slot_busy[s]) {
+            slot = s;
+            break;
+          }
+        if (slot >= 0) {
+          std::vector<uint64_t> ids;
+          while (!admitted.empty() && ids.size() < max_batch) {
+            ids.push_back(admitted.front());
+            admitted.pop_front();
+          }
+          const std::vector<uint64_t> batch_ids = ids;
+          try {
+            slot_busy[slot] = true;
+            if (!running.empty()) ++n_overlapped;
+            running.push_back(launch(slot, std::move(ids)));
+            ++n_batches;
+            batched_reqs += running.back().ids.size();
+            uint64_t live = 0;
+            for (auto &rb : running) live += rb.ids.size();
+            max_running_reqs = std::max(max_running_reqs, live);
+          } catch (const std::exception &e) {
+            // The stream may hold part of the batch: wait for it, then fail
+            // every request of the batch.
+            fatal = e.what();
+            (void)hipStreamSynchronize(streams[slot]);
+            Batch failed;
+            failed.slot = slot;
+            failed.ids = batch_ids;
+            complete(failed, HD_REQ_FAILED);
+            cv_done.notify_all();
+          }
+          continue;
+        }
+      }
+      if (stopping && running.empty()) return;
+      if (!running.empty()) cv_work.wait_for(lk, std::chrono::microseconds(20));
+      else cv_work.wait(lk);
+    }
+  }
+};
+
+extern "C" {
+
+int hd_server_create(const uint8_t *db, uint64_t n_db, uint64_t bytes_per_seq,
+                     uint64_t device_bytes, uint64_t host_bytes, int streams,
+                     uint64_t max_batch, hd_server **out) {
+  return guarded([&] {
+    HD_REQUIRE(out, "null output");
+    HD_REQUIRE(db && n_db > 0, "empty database");
+    HD_REQUIRE(bytes_per_seq > 0 && bytes_per_seq % 8 == 0,
+               "bytes_per_seq must be a positive multiple of 8");
+    HD_REQUIRE(streams >= 1 && streams <= 8, "streams must be 1..8");
+    HD_REQUIRE(device_bytes >= 4096 && host_bytes >= 4096, "budgets too small");
+    ensure_device();
+    auto s = std::make_unique<hd_server>();
+    s->n_db = n_db;
+    s->bpr = bytes_per_seq;
+    s->n_slots = streams;
+    s->max_batch = max_batch ? max_batch : 64;
+    s->db.alloc(n_db * bytes_per_seq);
+    HD_HIP(hipMemcpy(s->db.get(), db, n_db * bytes_per_seq, hipMemcpyHostToDevice));
+    s->dev_mem.alloc(device_bytes);
+    s->host_mem.alloc(host_bytes);
+    s->dev_arena.init(s->dev_mem.get<uint8_t>(), device_bytes);
+    s->host_arena.init(s->host_mem.get<uint8_t>(), host_bytes);
+    for (int k = 0; k < streams; ++k) {
+      hipStream_t st;
+      HD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      s->streams.push_back(st);
+      hipEvent_t ev;
+      HD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      s->events.push_back(ev);
+      s->slot_busy.push_back(false);
+    }
+    hd_server *raw = s.get();
+    s->worker = std::thread([raw] { raw->loop(); });
+    *out = s.release();
+  });
+}
+
+int hd_server_destroy(hd_server *s) {
+  return guarded([&] { delete s; });
+}
+
+static int submit(hd_server *s, std::unique_ptr<hd_server::Request> r, uint64_t *id) {
+  return guarded([&] {
+    HD_REQUIRE(s && id, "null server or id");
+    for (uint64_t row : r->rows) HD_REQUIRE(row < s->n_db, "row index out of range");
+    if (r->kind == hd_server::ALLPAIRS) r->g = geometry(r->m, s->bpr, r->semantics, 0xFFFFu);
+    r->lay = s->layout(*r);
+    if (r->lay.dev > s->dev_arena.size() || r->lay.host > s->host_arena.size())
+      throw Error(HD_ERR_CAPACITY, "request larger than the server's memory budget");
+    r->t_submit = std::chrono::steady_clock::now();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (!s->fatal.empty()) throw Error(HD_ERR_HIP, s->fatal);
+      r->id = s->next_id++;
+      r->state = HD_REQ_PENDING;
+      *id = r->id;
+      s->pending.push_back(r->id);
+      s->reqs[r->id] = std::move(r);
+    }
+    s->cv_work.notify_one();
+  });
+}
+
+int hd_server_submit_query(hd_server *s, const uint8_t *query, const uint64_t *rows,
+                           uint64_t m, int semantics, uint64_t *id) {
+  auto r = std::make_unique<hd_server::Request>();
+  if (!s || !query || !rows || m == 0 ||
+      (semantics != HD_SEM_DENSE && semantics != HD_SEM_CHAR_X)) {
+    set_last_error("query needs a server, a query, m >= 1 rows and DENSE or CHAR_X semantics");
+    return HD_ERR_INVALID;
+  }
+  r->kind = hd_server::QUERY;
+  r->semantics = semantics;
+  r->m = m;
+  r->query.assign(query, query + s->bpr);
+  r->rows.assign(rows, rows + m);
+  return submit(s, std::move(r), id);
+}
+
+int hd_server_submit_all_pairs(hd_server *s, const uint64_t *rows, uint64_t n, int semantics,
+                               int out_bytes, uint32_t max_dist, uint64_t *id) {
+  auto r = std::make_unique<hd_server::Request>();
+  if (!s || !rows || n == 0 || (out_bytes != 1 && out_bytes != 2 && out_bytes != 4) ||
+      (out_bytes < 4 && max_dist >= (1u << (8 * out_bytes)))) {
+    set_last_error("all-pairs job needs n >= 1 rows, out_bytes 1/2/4 and a fitting max_dist");
+    return HD_ERR_INVALID;
+  }
+  r->kind = hd_server::ALLPAIRS;
+  r->semantics = semantics;
+  r->m = n;
+  r->out_bytes = out_bytes;
+  r->max_dist = max_dist;
+  r->rows.assign(rows, rows + n);
+  return submit(s, std::move(r), id);
+}
+
+int hd_server_wait(hd_server *s, uint64_t id, double timeout_s, int *state) {
+  return guarded([&] {
+    HD_REQUIRE(s && state, "null server or state");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    HD_REQUIRE(it != s->reqs.end(), "unknown request");
+    auto done = [&] {
+      const int st = s->reqs[id]->state;
+      return st == HD_REQ_DONE || st == HD_REQ_FAILED;
+    };
+    if (timeout_s < 0) s->cv_done.wait(lk, done);
+    else s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done);
+    *state = s->reqs[id]->state;
+  });
+}
+
+int hd_server_result(hd_server *s, uint64_t id, void *out, uint64_t bytes, double *latency_s) {
+  return guarded([&] {
+    HD_REQUIRE(s, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    HD_REQUIRE(it != s->reqs.end(), "unknown request");
+    hd_server::Request &r = *it->second;
+    HD_REQUIRE(r.state == HD_REQ_DONE, "request not done");
+    HD_REQUIRE(bytes >= r.lay.out_bytes, "result buffer too small");
+    if (r.lay.out_bytes) {
+      HD_REQUIRE(out, "null buffer");
+      std::memcpy(out, s->host_arena.at(r.h_off) + r.lay.h_out, r.lay.out_bytes);
+    }
+    if (latency_s) *latency_s = std::chrono::duration<double>(r.t_done - r.t_submit).count();
+  });
+}
+
+int hd_server_release(hd_server *s, uint64_t id) {
+  return guarded([&] {
+    HD_REQUIRE(s, "null server");
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      auto it = s->reqs.find(id);
+      HD_REQUIRE(it != s->reqs.end(), "unknown request");
+      const int st = it->second->state;
+      HD_REQUIRE(st == HD_REQ_DONE || st == HD_REQ_FAILED, "request still in flight");
+      if (it->second->h_off != UINT64_MAX) s->host_arena.release(it->second->h_off);
+      s->reqs.erase(it);
+    }
+    s->cv_work.notify_one();
+  });
+}
+
+int hd_server_stats(hd_server *s, uint64_t *done, uint64_t *batches, uint64_t *overlapped,
+                    uint64_t *max_running, uint64_t *device_used) {
+  return guarded([&] {
+    HD_REQUIRE(s, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (done) *done = s->n_done;
+    if (batches) *batches = s->n_batches;
+    if (overlapped) *overlapped = s->n_overlapped;
+    if (max_running) *max_running = s->max_running_reqs;
+    if (device_used) *device_used = s->dev_arena.used();
+  });
+}
+
+} // extern "C"

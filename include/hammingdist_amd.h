@@ -199,6 +199,51 @@ int hd_batch_variant_count(void);
 int hd_batch_variant_info(int variant, int *rows_per_lane, int *cols_per_wave,
                           int *waves);
 
+/* ---- serving (no counterpart in the reference) --------------------------- */
+/* A packed database resident in device memory serves reference-distance
+ * queries against row subsets and small all-pairs jobs on its rows.  A
+ * worker thread admits requests in arrival order as soon as their device
+ * footprint fits the device budget (an arena the server owns, so no device
+ * allocation happens per request) and launches everything admitted so far
+ * as one batch on a free stream; up to `streams` batches run at once, so
+ * requests that arrive while others run overlap with them.  Results are
+ * kept in a pinned host arena until hd_server_release.  Each result equals
+ * the single-request entry point's (hd_reference_distances /
+ * hd_all_pairs) on the same rows.                                           */
+typedef struct hd_server hd_server;
+#define HD_REQ_PENDING 0
+#define HD_REQ_ADMITTED 1
+#define HD_REQ_RUNNING 2
+#define HD_REQ_DONE 3
+#define HD_REQ_FAILED 4
+/* db: n_db packed rows (host), copied to the device.  device_bytes and
+ * host_bytes bound the memory of requests in flight; max_batch bounds the
+ * requests of one launch (0 = 64).                                          */
+int hd_server_create(const uint8_t *db, uint64_t n_db, uint64_t bytes_per_seq,
+                     uint64_t device_bytes, uint64_t host_bytes, int streams,
+                     uint64_t max_batch, hd_server **out);
+int hd_server_destroy(hd_server *server);
+/* Distances (uint32) of query (bytes_per_seq bytes) to rows[0..m), in that
+ * order; semantics HD_SEM_DENSE or HD_SEM_CHAR_X.                          */
+int hd_server_submit_query(hd_server *server, const uint8_t *query,
+                           const uint64_t *rows, uint64_t m, int semantics,
+                           uint64_t *request);
+/* Lower triangle of the n database rows rows[0..n) (in that order).        */
+int hd_server_submit_all_pairs(hd_server *server, const uint64_t *rows,
+                               uint64_t n, int semantics, int out_bytes,
+                               uint32_t max_dist, uint64_t *request);
+/* Blocks until the request is done or failed, or timeout_s passes
+ * (timeout_s < 0: no limit); *state receives HD_REQ_*.                     */
+int hd_server_wait(hd_server *server, uint64_t request, double timeout_s,
+                   int *state);
+/* Copies a done request's result; *latency_s = completion - submission.   */
+int hd_server_result(hd_server *server, uint64_t request, void *out,
+                     uint64_t bytes, double *latency_s);
+int hd_server_release(hd_server *server, uint64_t request);
+int hd_server_stats(hd_server *server, uint64_t *done, uint64_t *batches,
+                    uint64_t *overlapped_batches, uint64_t *max_running,
+                    uint64_t *device_used);
+
 /* Number of all-pairs tile shapes compiled in, and a description of one.    */
 int hd_variant_count(void);
 int hd_variant_info(int variant, int *rows_per_lane, int *cols_per_wave,

@@ -1,0 +1,71 @@
+"""Serving loop (hd_server_*): every result equals the oracle's, and the same
+request gives the same result alone and in a concurrent mix, including when
+a small device budget forces admission to wait for memory."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from hammingdist_amd import capi, serving, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def db():
+    spec = workloads.large(n=4000, L=3001, seed=77)
+    return workloads.generate_packed(spec), spec.L
+
+
+def small_requests(db, L, count, seed):
+    return serving.make_requests(db, L, count, seed, query_frac=0.7, m_min=50, m_max=4000,
+                                 n_min=2, n_max=300, p_sub=3e-3)
+
+
+def oracle_result(db, r):
+    if r.kind == "query":
+        return np.array([O.distance_dense(r.query, db[int(k)]) for k in r.rows], np.uint32)
+    return O.all_pairs_dense(np.ascontiguousarray(db[r.rows.astype(np.int64)]), r.out_bytes,
+                             r.max_dist)
+
+
+def test_results_match_oracle_and_are_identical_alone_and_mixed(db):
+    packed, L = db
+    reqs = small_requests(packed, L, 60, seed=5)
+    srv = capi.Server(packed, device_bytes=1 << 30, host_bytes=256 << 20, streams=3)
+    alone = serving.run_alone(srv, reqs)
+    for r, a in zip(reqs, alone):
+        np.testing.assert_array_equal(a, oracle_result(packed, r))
+    mixed, secs, lat = serving.closed_loop(srv, reqs)
+    for a, b in zip(alone, mixed):
+        np.testing.assert_array_equal(a, b)
+    st = srv.stats()
+    assert st["done"] == 120
+    assert st["batches"] < 120  # requests were batched together
+    open_res, _, _ = serving.open_loop(srv, reqs, rate=2000.0, seed=3)
+    for a, b in zip(alone, open_res):
+        np.testing.assert_array_equal(a, b)
+    srv.close()
+
+
+def test_small_budget_queues_by_footprint(db):
+    packed, L = db
+    reqs = small_requests(packed, L, 30, seed=9)
+    # room for only a few requests at a time
+    biggest = max(r.rows.size for r in reqs)
+    budget = max(4 << 20, biggest * (packed.shape[1] + 64) * 3)
+    srv = capi.Server(packed, device_bytes=budget, host_bytes=64 << 20, streams=2)
+    mixed, _, _ = serving.closed_loop(srv, reqs)
+    for r, a in zip(reqs, mixed):
+        np.testing.assert_array_equal(a, oracle_result(packed, r))
+    srv.close()
+
+
+def test_oversized_request_is_rejected(db):
+    packed, L = db
+    srv = capi.Server(packed, device_bytes=1 << 20, host_bytes=1 << 20, streams=1)
+    rows = np.arange(3000, dtype=np.uint64)
+    with pytest.raises(capi.EngineError, match="budget"):
+        srv.submit_all_pairs(rows)
+    with pytest.raises(capi.EngineError, match="out of range"):
+        srv.submit_query(packed[0], np.array([packed.shape[0]], np.uint64))
+    srv.close()
