@@ -108,6 +108,22 @@ def cpu_baseline(packed, L: int, target_s: float = 12.0):
     }
 
 
+def measured_traffic(n: int, L: int, md: int):
+    """HBM bytes per launch of the headline kernel from the committed
+    rocprofv3 counter summary of the same command (FETCH_SIZE and WRITE_SIZE
+    per dispatch, each scaled by the same counter's reading for a copy of
+    known size); None for any other configuration."""
+    path = os.path.join(ROOT, "profiles", "k_allpairs_dense_large50k.json")
+    try:
+        with open(path) as f:
+            p = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (p.get("n"), p.get("L"), p.get("max_distance")) != (n, L, md):
+        return None
+    return p.get("traffic_bytes")
+
+
 def tri_row(x: int) -> int:
     i = int((1 + (1 + 8 * x) ** 0.5) // 2)
     while i * (i - 1) // 2 > x:
@@ -211,7 +227,7 @@ def run_large(args, torch, capi, workloads, np):
             "peak": PEAK_INT32_OPS / 1e12,
             "unit": "TOP/s",
             "frac": achieved / PEAK_INT32_OPS,
-            "traffic": None,
+            "traffic": measured_traffic(n, args.L, md),
             "kernel": "k_allpairs<4,4,16,4,uint8> (dense, 4 planes)",
             "kernel_ms": kern_ms,
             "ops_per_launch": ops,
@@ -432,6 +448,8 @@ def main() -> int:
     ap.add_argument("--no-wide", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only a 1 GiB device copy (counter calibration)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline step (for profiling)")
     args = ap.parse_args()
@@ -442,6 +460,18 @@ def main() -> int:
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         return 2
+    if args.copy_probe:
+        # A plain 1 GiB device-to-device copy, three times: the known-size
+        # kernel that scales the FETCH_SIZE / WRITE_SIZE counters.
+        import torch
+
+        x = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": x.numel() * 4, "copies": 3}))
+        return 0
 
     import numpy as np
     import torch
