@@ -380,6 +380,54 @@ def run_batched(args, torch, capi, workloads, np):
     return res
 
 
+def run_e2e(args, workloads, np):
+    """from_fasta_to_lower_triangular end to end on the large workload: FASTA
+    file in, lower-triangle text file out (max_distance 2), the reference's
+    own published benchmark (plots/speed.txt:12, n = 50,000)."""
+    import tempfile
+
+    import hammingdist_amd as hd
+
+    spec = workloads.large(n=args.n, L=args.L)
+    tmp = tempfile.mkdtemp(prefix="hd_e2e_")
+    fasta, out = os.path.join(tmp, "large.fa"), os.path.join(tmp, "lt.txt")
+    try:
+        t = time.perf_counter()
+        workloads.write_fasta(spec, fasta)
+        log(f"e2e: wrote FASTA ({os.path.getsize(fasta) / 1e9:.2f} GB) in "
+            f"{time.perf_counter() - t:.1f}s")
+        t = time.perf_counter()
+        hd.from_fasta_to_lower_triangular(fasta, out, max_distance=2)
+        secs = time.perf_counter() - t
+        size = os.path.getsize(out)
+        n = spec.n
+        assert size == n * (n - 1), "every value is one digit: row i has 2i bytes"
+        # rows checked in place: row i starts at byte i(i-1)
+        O = oracle()
+        packed = workloads.generate_packed(spec)
+        rng = np.random.default_rng(2)
+        with open(out, "rb") as f:
+            for i in [1, 2, 3, n - 1] + [int(x) for x in rng.integers(4, n - 1, 6)]:
+                f.seek(i * (i - 1))
+                line = f.read(2 * i).decode()
+                assert line.endswith("\n")
+                vals = [int(v) for v in line[:-1].split(",")]
+                for j in rng.integers(0, i, 20):
+                    assert vals[int(j)] == O.distance_dense(packed[i], packed[int(j)], 2)
+        cells = n * (n - 1) // 2
+        res = {"seconds": secs, "cells_per_s": cells / secs, "text_GB": size / 1e9,
+               "reference_a100_seconds": 20.969, "speedup_vs_reference_a100": 20.969 / secs,
+               "reference_52core_cpu_seconds": 59.748}
+        log(f"e2e: from_fasta_to_lower_triangular n={n} in {secs:.2f}s "
+            f"({size / 1e9:.2f} GB text)")
+        return res
+    finally:
+        for p in (fasta, out):
+            if os.path.exists(p):
+                os.remove(p)
+        os.rmdir(tmp)
+
+
 def run_serving(args, torch, capi, workloads, np):
     from hammingdist_amd import serving
 
@@ -448,6 +496,7 @@ def main() -> int:
     ap.add_argument("--no-wide", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only a 1 GiB device copy (counter calibration)")
     ap.add_argument("--headline-only", action="store_true",
@@ -455,7 +504,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.headline_only:
         args.no_cpu_baseline = args.no_early_exit_run = args.no_refdist = True
-        args.no_wide = args.no_batched = args.no_serving = True
+        args.no_wide = args.no_batched = args.no_serving = args.no_e2e = True
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -505,6 +554,8 @@ def main() -> int:
         result["batched"] = run_batched(args, torch, capi, workloads, np)
     if not args.no_serving:
         result["serving"] = run_serving(args, torch, capi, workloads, np)
+    if not args.no_e2e:
+        result["e2e_stream"] = run_e2e(args, workloads, np)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(packed, args.L)

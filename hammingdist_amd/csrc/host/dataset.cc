@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 
@@ -230,20 +231,21 @@ namespace {
 struct StreamWriter {
   std::string filename;
   int out_bytes;
-  bool started = false;
+  std::uint32_t max_value;
+  std::unique_ptr<RowWriter> writer;
 };
 
 int write_band(void *user, const void *cells, std::uint64_t, std::uint64_t,
                std::uint64_t row_begin, std::uint64_t row_end) {
   auto *w = static_cast<StreamWriter *>(user);
   try {
+    if (!w->writer) w->writer = std::make_unique<RowWriter>(w->filename, false);
     if (w->out_bytes == 1)
-      write_rows(w->filename, static_cast<const std::uint8_t *>(cells), row_begin, row_end,
-                 w->started);
+      w->writer->write(static_cast<const std::uint8_t *>(cells), row_begin, row_end,
+                       w->max_value);
     else
-      write_rows(w->filename, static_cast<const std::uint16_t *>(cells), row_begin, row_end,
-                 w->started);
-    w->started = true;
+      w->writer->write(static_cast<const std::uint16_t *>(cells), row_begin, row_end,
+                       w->max_value);
     return 0;
   } catch (const std::exception &e) {
     std::cerr << "# hammingdist :: write failed: " << e.what() << std::endl;
@@ -277,7 +279,7 @@ void from_fasta_to_lower_triangular(const std::string &input_filename,
   const std::uint16_t max_dist = max_distance > 65535 ? std::uint16_t{65535}
                                                       : static_cast<std::uint16_t>(max_distance);
   // The text is the same for any cell width that holds max_dist.
-  StreamWriter w{output_filename, max_dist <= 255 ? 1 : 2};
+  StreamWriter w{output_filename, max_dist <= 255 ? 1 : 2, max_dist, nullptr};
   const int st = hd_all_pairs_stream(packed.data(), count, bps, HD_SEM_DENSE, w.out_bytes,
                                      max_dist, 0, write_band, &w);
   check(st);

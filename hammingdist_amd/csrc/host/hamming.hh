@@ -108,6 +108,28 @@ std::vector<ReferenceDistIntType> fasta_reference_distances(
     bool include_x = false);
 
 // ---- text output (include/hamming/hamming_utils.hh:14-110) ----------------
+// Writes whole rows of a lower triangle to one file, appending row bands in
+// order: each band is formatted in parallel chunks, and the chunks are
+// written in parallel at their final offsets.
+class RowWriter {
+public:
+  RowWriter(const std::string &filename, bool append);
+  ~RowWriter();
+  RowWriter(const RowWriter &) = delete;
+  RowWriter &operator=(const RowWriter &) = delete;
+  // Rows [row_begin, row_end); `cells` starts at (row_begin, 0); every value
+  // is <= max_value (values below 10 take a one-digit path).
+  template <typename T>
+  void write(const T *cells, std::uint64_t row_begin, std::uint64_t row_end,
+             std::uint32_t max_value);
+  void write_raw(const char *data, std::size_t len);
+
+private:
+  std::string filename_;
+  int fd_ = -1;
+  std::uint64_t offset_ = 0;
+};
+
 // Writes whole rows [row_begin, row_end) of a triangle whose first cell in
 // `cells` is (row_begin, 0); truncates the file when `append` is false.
 template <typename T>

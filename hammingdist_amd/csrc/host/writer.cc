@@ -1,6 +1,9 @@
 // Text output of the host layer: lower-triangular rows and sparse triplets.
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
+#include <fcntl.h>
+#include <unistd.h>
 #include <cstring>
 #include <fstream>
 #include <memory>
@@ -61,30 +64,72 @@ std::size_t format_rows(const T *cells, std::uint64_t r0, std::uint64_t r1, char
   return static_cast<std::size_t>(p - out);
 }
 
-void write_all(std::FILE *f, const char *data, std::size_t len) {
-  if (len && std::fwrite(data, 1, len, f) != len)
-    throw std::runtime_error("write to lower-triangular file failed");
+// Rows [r0, r1) whose cells are all single digits: row i is exactly 2i bytes.
+template <typename T>
+std::size_t format_rows_1digit(const T *cells, std::uint64_t r0, std::uint64_t r1, char *out) {
+  char *p = out;
+  const T *c = cells;
+  for (std::uint64_t i = r0; i < r1; ++i) {
+    for (std::uint64_t j = 0; j < i; ++j) {
+      p[0] = static_cast<char>('0' + *c++);
+      p[1] = ',';
+      p += 2;
+    }
+    if (i > 0) p[-1] = '\n';
+  }
+  return static_cast<std::size_t>(p - out);
+}
+
+void pwrite_all(int fd, const char *data, std::size_t len, std::uint64_t off) {
+  while (len) {
+    const ssize_t w = ::pwrite(fd, data, len, static_cast<off_t>(off));
+    if (w < 0) {
+      if (errno == EINTR) continue;
+      throw std::runtime_error(std::string("write to lower-triangular file failed: ") +
+                               std::strerror(errno));
+    }
+    data += w;
+    len -= static_cast<std::size_t>(w);
+    off += static_cast<std::uint64_t>(w);
+  }
 }
 
 } // namespace
 
+RowWriter::RowWriter(const std::string &filename, bool append) : filename_(filename) {
+  // No O_APPEND even when appending: on Linux pwrite on an O_APPEND
+  // descriptor ignores its offset, so the end is read once instead.
+  fd_ = ::open(filename.c_str(), O_WRONLY | O_CREAT | (append ? 0 : O_TRUNC), 0644);
+  if (fd_ < 0) throw std::runtime_error("Error: Failed to open file '" + filename + "'");
+  if (append) {
+    const off_t end = ::lseek(fd_, 0, SEEK_END);
+    offset_ = end > 0 ? static_cast<std::uint64_t>(end) : 0;
+  }
+}
+
+RowWriter::~RowWriter() {
+  if (fd_ >= 0) ::close(fd_);
+}
+
+void RowWriter::write_raw(const char *data, std::size_t len) {
+  pwrite_all(fd_, data, len, offset_);
+  offset_ += len;
+}
+
+// Rows [row_begin, row_end) of a triangle whose first cell in `cells` is
+// (row_begin, 0): chunks of about equal cell counts are formatted in
+// parallel, then written in parallel at their prefix-summed file offsets.
 template <typename T>
-void write_rows(const std::string &filename, const T *cells, std::uint64_t row_begin,
-                std::uint64_t row_end, bool append) {
+void RowWriter::write(const T *cells, std::uint64_t row_begin, std::uint64_t row_end,
+                      std::uint32_t max_value) {
   if (row_begin == 0) row_begin = 1; // row 0 has no cells
   if (row_end <= row_begin) return;
-  std::FILE *f = std::fopen(filename.c_str(), append ? "ab" : "wb");
-  if (!f) throw std::runtime_error("Error: Failed to open file '" + filename + "'");
-  std::unique_ptr<std::FILE, int (*)(std::FILE *)> guard(f, std::fclose);
-  // Chunks of rows with about equal cell counts, formatted in parallel and
-  // written in row order.
   const std::uint64_t c0 = row_begin * (row_begin - 1) / 2;
   const std::uint64_t total = row_end * (row_end - 1) / 2 - c0;
   const std::uint64_t chunks = std::max<std::uint64_t>(
       1, std::min<std::uint64_t>(worker_count() * 4, total / (1u << 16) + 1));
   std::vector<std::uint64_t> bounds{row_begin};
   for (std::uint64_t k = 1; k < chunks; ++k) {
-    // first row whose start cell passes k/chunks of the band
     const double target = static_cast<double>(c0) + static_cast<double>(total) * k / chunks;
     std::uint64_t r = static_cast<std::uint64_t>(0.5 + std::sqrt(2.0 * target + 0.25));
     r = std::max(r, bounds.back());
@@ -93,19 +138,42 @@ void write_rows(const std::string &filename, const T *cells, std::uint64_t row_b
   }
   bounds.push_back(row_end);
   const std::size_t nb = bounds.size() - 1;
+  const bool one_digit = max_value < 10;
   std::vector<std::string> text(nb);
   parallel_for(nb, [&](std::uint64_t b, std::uint64_t e) {
     for (std::uint64_t k = b; k < e; ++k) {
       const std::uint64_t r0 = bounds[k], r1 = bounds[k + 1];
       if (r1 <= r0) continue;
       const std::uint64_t cells_k = r1 * (r1 - 1) / 2 - r0 * (r0 - 1) / 2;
-      text[k].resize(cells_k * (kMaxDigits + 1) + 1);
-      const std::size_t len =
-          format_rows(cells + (r0 * (r0 - 1) / 2 - c0), r0, r1, text[k].data());
-      text[k].resize(len);
+      const T *src = cells + (r0 * (r0 - 1) / 2 - c0);
+      if (one_digit) {
+        text[k].resize(2 * cells_k);
+        text[k].resize(format_rows_1digit(src, r0, r1, text[k].data()));
+      } else {
+        text[k].resize(cells_k * (kMaxDigits + 1) + 1);
+        text[k].resize(format_rows(src, r0, r1, text[k].data()));
+      }
     }
   });
-  for (const auto &t : text) write_all(f, t.data(), t.size());
+  std::vector<std::uint64_t> off(nb + 1, 0);
+  for (std::size_t k = 0; k < nb; ++k) off[k + 1] = off[k] + text[k].size();
+  parallel_for(nb, [&](std::uint64_t b, std::uint64_t e) {
+    for (std::uint64_t k = b; k < e; ++k)
+      pwrite_all(fd_, text[k].data(), text[k].size(), offset_ + off[k]);
+  });
+  offset_ += off[nb];
+}
+
+template void RowWriter::write<std::uint8_t>(const std::uint8_t *, std::uint64_t, std::uint64_t,
+                                             std::uint32_t);
+template void RowWriter::write<std::uint16_t>(const std::uint16_t *, std::uint64_t,
+                                              std::uint64_t, std::uint32_t);
+
+template <typename T>
+void write_rows(const std::string &filename, const T *cells, std::uint64_t row_begin,
+                std::uint64_t row_end, bool append) {
+  RowWriter w(filename, append);
+  w.write(cells, row_begin, row_end, 0xFFFFFFFFu);
 }
 
 template void write_rows<std::uint8_t>(const std::string &, const std::uint8_t *,
@@ -122,15 +190,12 @@ void write_lower_triangular(const std::string &filename, const std::vector<T> &c
   if (size == 0) return;
   std::uint64_t rows = 1; // full rows 1..rows-1
   while ((rows + 1) * rows / 2 <= size) ++rows;
-  write_rows(filename, cells.data(), 1, rows, false);
+  std::uint32_t mx = 0;
+  for (const T v : cells) mx = std::max<std::uint32_t>(mx, v);
+  RowWriter w(filename, false);
+  w.write(cells.data(), 1, rows, mx);
   const std::uint64_t used = rows * (rows - 1) / 2;
   if (used < size) {
-    if (rows == 1) {
-      std::FILE *f = std::fopen(filename.c_str(), "wb");
-      if (f) std::fclose(f);
-    }
-    std::FILE *f = std::fopen(filename.c_str(), "ab");
-    if (!f) throw std::runtime_error("Error: Failed to open file '" + filename + "'");
     std::string tail;
     char buf[16];
     for (std::uint64_t x = used; x < size; ++x) {
@@ -138,8 +203,7 @@ void write_lower_triangular(const std::string &filename, const std::vector<T> &c
       tail.append(buf, static_cast<std::size_t>(e - buf));
       tail.push_back(',');
     }
-    write_all(f, tail.data(), tail.size());
-    std::fclose(f);
+    w.write_raw(tail.data(), tail.size());
   }
 }
 
@@ -189,7 +253,9 @@ void DataSet<T>::dump_sparse(const std::string &filename, int threshold) const {
       s.resize(static_cast<std::size_t>(p - s.data()));
     }
   });
-  for (const auto &t : text) write_all(f, t.data(), t.size());
+  for (const auto &t : text)
+    if (!t.empty() && std::fwrite(t.data(), 1, t.size(), f) != t.size())
+      throw std::runtime_error("write to sparse file failed");
 }
 
 template void DataSet<std::uint8_t>::dump_sparse(const std::string &, int) const;

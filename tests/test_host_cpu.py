@@ -121,6 +121,25 @@ def test_lower_triangular_roundtrip_and_writer(tmp_path, n, rng):
             ds[n, 0]
 
 
+@pytest.mark.parametrize("top", [2, 10, 300])
+def test_writer_many_chunks(tmp_path, top, rng):
+    # 700 rows = 244,650 cells: several parallel chunks; top <= 10 takes the
+    # one-digit formatting path.
+    n = 700
+    cells = rng.integers(0, top, size=n * (n - 1) // 2).astype(np.uint16)
+    src = str(tmp_path / "lt.txt")
+    O.lib().oracle_write_lower_triangular(src.encode(), cells.ctypes.data, n, cells.itemsize)
+    ds = hd.from_lower_triangular_large(src)
+    np.testing.assert_array_equal(ds.lt_array, cells)
+    out = str(tmp_path / "out.txt")
+    ds.dump_lower_triangular(out)
+    assert open(out, "rb").read() == open(src, "rb").read()
+    # a second dump over an existing, longer file truncates it
+    open(out, "ab").write(b"stale tail\n" * 1000)
+    ds.dump_lower_triangular(out)
+    assert open(out, "rb").read() == open(src, "rb").read()
+
+
 def test_dump_sequence_indices_without_dedup(tmp_path):
     src = str(tmp_path / "lt.txt")
     lt_file(src, [[1], [2, 3]])
