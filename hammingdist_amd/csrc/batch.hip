@@ -27,17 +27,20 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // One dataset of a batch, as the kernels see it.
 struct BatchJobDev {
   const uint32_t *planes;
+  const uint32_t *corr; // dense 3-op code: Z[n_pad], G[n_pad]
   void *out;
   uint64_t tile_begin; // first index of this job in the batch's tile list
-  uint32_t n, W, P, out_bytes, max_dist, check_every;
+  uint32_t n, n_pad, W, P, out_bytes, max_dist, check_every;
 };
 
 struct EncodeJobDev {
   const uint8_t *packed;
   uint32_t *planes;
+  uint32_t *corr;       // dense 3-op code: Z[n_pad], G[n_pad]
   uint64_t bpr;
   uint64_t block_begin; // first encode block of this job
-  uint32_t n, n_pad, W, P, runs, identity;
+  uint64_t count_begin; // first count block (row group) of this job
+  uint32_t n, n_pad, W, P, runs, mode;
   Table16 tbl;
 };
 
@@ -74,17 +77,26 @@ __global__ __launch_bounds__(256) void k_batch_encode(const EncodeJobDev *__rest
   const uint32_t i = (uint32_t)(local % row_blocks) * 256 + threadIdx.x;
   const uint32_t run = (uint32_t)(local / row_blocks);
   encode_run(J.packed, J.n, J.bpr, J.W, (int)J.P, s_tbl, J.planes, i, run * kEncodeRun,
-             J.identity != 0);
+             (int)J.mode);
+}
+
+// Z and G counts of every dense 3-op job: one workgroup per row group.
+__global__ __launch_bounds__(256) void k_batch_counts(const EncodeJobDev *__restrict__ jobs,
+                                                      uint32_t n_jobs) {
+  const uint32_t j = find_job(jobs, n_jobs, blockIdx.x, &EncodeJobDev::count_begin);
+  const EncodeJobDev &J = jobs[j];
+  dense3_counts_group(J.planes, J.W, (uint32_t)(blockIdx.x - J.count_begin), J.n_pad, J.corr);
 }
 
 __global__ void k_reset_counter(unsigned long long *counter) { *counter = 0; }
 
 // Persistent workgroups take tiles from the batch's list (largest per-tile
 // work first) through one atomic counter until the list is exhausted; each
-// tile is computed with its job's plane count and cell type.  ONLY4: every
-// job has 4 planes (the dense code), so only that tile loop is compiled in
-// and the kernel keeps the register budget of the plain kernel.
-template <int R, int C, bool ONLY4>
+// tile is computed with its job's plane code and cell type.  ONLY (kDense3
+// or 4): every job has that plane code, so only its tile loop is compiled in
+// and the kernel keeps the register budget of the plain kernel; ONLY = 0
+// handles any mix.
+template <int R, int C, int ONLY>
 __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_allpairs(
     const BatchJobDev *__restrict__ jobs, uint32_t n_jobs, uint64_t total_tiles,
     unsigned long long *__restrict__ counter) {
@@ -109,8 +121,11 @@ __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_allpairs(
     constexpr int CW = C > 8 ? 8 : C;
 #define HD_BATCH_TILE(PP, CC, JJ)                                                        \
   tile_wave<PP, R, CC>(J.planes, J.n, J.W, i0, JJ, J.out, J.out_bytes, 0, J.max_dist,     \
-                       J.check_every);
-    switch (ONLY4 ? 4u : J.P) {
+                       J.check_every, J.corr, J.n_pad);
+    switch (ONLY ? (uint32_t)ONLY : J.P) {
+    case kDense3:
+      HD_BATCH_TILE(kDense3, C, j0)
+      break;
     case 4:
       HD_BATCH_TILE(4, C, j0)
       break;
@@ -131,7 +146,7 @@ __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_allpairs(
 
 // The same tiles as one grid: workgroup g computes tile g of the batch's list
 // (the hardware dispatcher balances the load; the list is in LPT order).
-template <int R, int C, bool ONLY4>
+template <int R, int C, int ONLY>
 __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_grid(
     const BatchJobDev *__restrict__ jobs, uint32_t n_jobs, uint64_t tile_begin) {
   constexpr uint32_t TI = kWave * R;
@@ -144,49 +159,59 @@ __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_grid(
   tile_coords(t - J.tile_begin, TI / TJ, b, cb);
   const uint32_t i0 = b * TI, j0 = cb * TJ + wave * C;
   constexpr int CW = C > 8 ? 8 : C;
-  if (ONLY4 || J.P == 4) {
+  const uint32_t P = ONLY ? (uint32_t)ONLY : J.P;
+  if (P == kDense3) {
+    tile_wave<kDense3, R, C>(J.planes, J.n, J.W, i0, j0, J.out, J.out_bytes, 0, J.max_dist,
+                             J.check_every, J.corr, J.n_pad);
+  } else if (P == 4) {
     tile_wave<4, R, C>(J.planes, J.n, J.W, i0, j0, J.out, J.out_bytes, 0, J.max_dist,
-                       J.check_every);
-  } else if (J.P == 5) {
+                       J.check_every, nullptr, J.n_pad);
+  } else if (P == 5) {
     for (uint32_t h = 0; h < (uint32_t)(C / CW); ++h)
       tile_wave<5, R, CW>(J.planes, J.n, J.W, i0, j0 + h * CW, J.out, J.out_bytes, 0,
-                          J.max_dist, J.check_every);
+                          J.max_dist, J.check_every, nullptr, J.n_pad);
   } else {
     for (uint32_t h = 0; h < (uint32_t)(C / CW); ++h)
       tile_wave<6, R, CW>(J.planes, J.n, J.W, i0, j0 + h * CW, J.out, J.out_bytes, 0,
-                          J.max_dist, J.check_every);
+                          J.max_dist, J.check_every, nullptr, J.n_pad);
   }
 }
 
-template <int V> static const void *grid_kernel_v(bool only4) {
-  return only4 ? reinterpret_cast<const void *>(&k_batch_grid<kShapes[V].R, kShapes[V].C, true>)
-               : reinterpret_cast<const void *>(&k_batch_grid<kShapes[V].R, kShapes[V].C, false>);
-}
-
-static const void *grid_kernel_of(int v, bool only4) {
-  switch (v) {
-  case 0: return grid_kernel_v<0>(only4);
-  case 1: return grid_kernel_v<1>(only4);
-  case 2: return grid_kernel_v<2>(only4);
-  case 3: return grid_kernel_v<3>(only4);
-  default: return grid_kernel_v<4>(only4);
+template <int V> static const void *grid_kernel_v(int only) {
+  constexpr int R = kShapes[V].R, C = kShapes[V].C;
+  switch (only) {
+  case kDense3: return reinterpret_cast<const void *>(&k_batch_grid<R, C, kDense3>);
+  case 4: return reinterpret_cast<const void *>(&k_batch_grid<R, C, 4>);
+  default: return reinterpret_cast<const void *>(&k_batch_grid<R, C, 0>);
   }
 }
 
-template <int V> static const void *batch_kernel_v(bool only4) {
-  return only4 ? reinterpret_cast<const void *>(
-                     &k_batch_allpairs<kShapes[V].R, kShapes[V].C, true>)
-               : reinterpret_cast<const void *>(
-                     &k_batch_allpairs<kShapes[V].R, kShapes[V].C, false>);
+static const void *grid_kernel_of(int v, int only) {
+  switch (v) {
+  case 0: return grid_kernel_v<0>(only);
+  case 1: return grid_kernel_v<1>(only);
+  case 2: return grid_kernel_v<2>(only);
+  case 3: return grid_kernel_v<3>(only);
+  default: return grid_kernel_v<4>(only);
+  }
 }
 
-static const void *batch_kernel_of(int v, bool only4) {
+template <int V> static const void *batch_kernel_v(int only) {
+  constexpr int R = kShapes[V].R, C = kShapes[V].C;
+  switch (only) {
+  case kDense3: return reinterpret_cast<const void *>(&k_batch_allpairs<R, C, kDense3>);
+  case 4: return reinterpret_cast<const void *>(&k_batch_allpairs<R, C, 4>);
+  default: return reinterpret_cast<const void *>(&k_batch_allpairs<R, C, 0>);
+  }
+}
+
+static const void *batch_kernel_of(int v, int only) {
   switch (v) {
-  case 0: return batch_kernel_v<0>(only4);
-  case 1: return batch_kernel_v<1>(only4);
-  case 2: return batch_kernel_v<2>(only4);
-  case 3: return batch_kernel_v<3>(only4);
-  default: return batch_kernel_v<4>(only4);
+  case 0: return batch_kernel_v<0>(only);
+  case 1: return batch_kernel_v<1>(only);
+  case 2: return batch_kernel_v<2>(only);
+  case 3: return batch_kernel_v<3>(only);
+  default: return batch_kernel_v<4>(only);
   }
 }
 
@@ -211,9 +236,9 @@ struct hd_batch {
   DeviceBuffer d_jobs, d_enc, d_counter;
   std::vector<BatchJobDev> host_jobs;
   uint32_t n_jobs_active = 0;
-  uint64_t total_tiles = 0, enc_blocks = 0;
+  uint64_t total_tiles = 0, enc_blocks = 0, count_blocks = 0;
   int variant = -1, early_exit = 1;
-  bool only4 = true; // every job encodes to 4 planes
+  int only = 4; // the plane code every job shares (kDense3 or 4), else 0
   uint32_t persistent_blocks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -239,6 +264,12 @@ void enqueue_batch(hd_batch *b, hipStream_t s) {
                        b->d_enc.get<EncodeJobDev>(), (uint32_t)b->jobs.size());
     HD_HIP(hipGetLastError());
   }
+  if (b->count_blocks) {
+    HD_REQUIRE(b->count_blocks < (1ull << 31), "batch too large for one count launch");
+    hipLaunchKernelGGL(k_batch_counts, dim3((uint32_t)b->count_blocks), dim3(256), 0, s,
+                       b->d_enc.get<EncodeJobDev>(), (uint32_t)b->jobs.size());
+    HD_HIP(hipGetLastError());
+  }
   if (!b->total_tiles) return;
   const int shape = b->variant % kNumShapes;
   if (b->variant >= kNumShapes) {
@@ -248,7 +279,7 @@ void enqueue_batch(hd_batch *b, hipStream_t s) {
       const uint64_t cnt = std::min<uint64_t>(0x7fffffffull, b->total_tiles - t0);
       uint64_t tb = t0;
       void *gargs[] = {(void *)&jp, (void *)&nj, (void *)&tb};
-      HD_HIP(hipLaunchKernel(grid_kernel_of(shape, b->only4), dim3((uint32_t)cnt),
+      HD_HIP(hipLaunchKernel(grid_kernel_of(shape, b->only), dim3((uint32_t)cnt),
                              dim3(kWave * kBatchWaves), gargs, 0, s));
     }
     return;
@@ -261,7 +292,7 @@ void enqueue_batch(hd_batch *b, hipStream_t s) {
   uint64_t tt = b->total_tiles;
   unsigned long long *ctr = b->d_counter.get<unsigned long long>();
   void *args[] = {(void *)&jp, (void *)&nj, (void *)&tt, (void *)&ctr};
-  HD_HIP(hipLaunchKernel(batch_kernel_of(shape, b->only4), dim3(b->persistent_blocks),
+  HD_HIP(hipLaunchKernel(batch_kernel_of(shape, b->only), dim3(b->persistent_blocks),
                          dim3(kWave * kBatchWaves), args, 0, s));
 }
 
@@ -285,41 +316,49 @@ void prepare(hd_batch *b, int variant, int early_exit) {
   });
   b->host_jobs.clear();
   std::vector<EncodeJobDev> enc;
-  uint64_t tiles = 0, blocks = 0;
+  uint64_t tiles = 0, blocks = 0, count_blocks = 0;
   for (size_t k : order) {
     auto &J = *b->jobs[k];
     EncodeJobDev e{};
     e.packed = J.packed.get<uint8_t>();
     e.planes = J.planes.get<uint32_t>();
+    e.corr = is_dense3(J.g.code) ? e.planes + J.g.plane_words() : nullptr;
     e.bpr = J.bpr;
     e.block_begin = blocks;
+    e.count_begin = count_blocks;
     e.n = (uint32_t)J.n;
     e.n_pad = (uint32_t)J.g.n_pad;
     e.W = J.g.W;
     e.P = (uint32_t)J.g.code.planes;
     e.runs = (J.g.W + kEncodeRun - 1) / kEncodeRun;
     std::memcpy(e.tbl.v, J.g.code.table, 16);
-    e.identity = is_identity(J.g.code) ? 1u : 0u;
+    e.mode = (uint32_t)encode_mode(J.g.code);
     blocks += (uint64_t)(e.n_pad / 256) * e.runs;
+    if (is_dense3(J.g.code)) count_blocks += e.n_pad / 64;
     enc.push_back(e);
     if (J.n < 2) continue;
     BatchJobDev d{};
     d.planes = J.planes.get<uint32_t>();
+    d.corr = e.corr;
     d.out = J.out.get();
     d.tile_begin = tiles;
     d.n = (uint32_t)J.n;
+    d.n_pad = (uint32_t)J.g.n_pad;
     d.W = J.g.W;
-    d.P = (uint32_t)J.g.code.planes;
+    d.P = (uint32_t)J.g.code.op;
     d.out_bytes = (uint32_t)J.out_bytes;
     d.max_dist = J.max_dist;
     d.check_every = check_interval(early_exit != 0, J.max_dist, J.g.W);
     tiles += tiles_before((J.n + TI - 1) / TI, TI / TJ);
     b->host_jobs.push_back(d);
   }
-  b->only4 = true;
-  for (const auto &d : b->host_jobs) b->only4 = b->only4 && d.P == 4;
+  b->only = b->host_jobs.empty() ? 4 : (int)b->host_jobs[0].P;
+  for (const auto &d : b->host_jobs)
+    if ((int)d.P != b->only) b->only = 0;
+  if (b->only != kDense3 && b->only != 4) b->only = 0;
   b->total_tiles = tiles;
   b->enc_blocks = blocks;
+  b->count_blocks = count_blocks;
   b->n_jobs_active = (uint32_t)b->host_jobs.size();
   b->d_jobs.alloc(std::max<size_t>(1, b->host_jobs.size()) * sizeof(BatchJobDev));
   b->d_enc.alloc(enc.size() * sizeof(EncodeJobDev));
@@ -337,13 +376,13 @@ void prepare(hd_batch *b, int variant, int early_exit) {
   HD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   HD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
                                                       batch_kernel_of(b->variant % kNumShapes,
-                                                                      b->only4),
+                                                                      b->only),
                                                       kWave * kBatchWaves, 0));
   b->persistent_blocks = (uint32_t)std::max<uint64_t>(
       1, std::min<uint64_t>((uint64_t)cus * std::max(per_cu, 1), tiles));
 
-  // Capture: batched encode -> counter reset (a kernel, not a memset) ->
-  // persistent all-pairs.
+  // Capture: batched encode -> dense 3-op counts -> counter reset (a kernel,
+  // not a memset) -> persistent all-pairs.
   Stream cap;
   HD_HIP(hipStreamBeginCapture(cap.get(), hipStreamCaptureModeThreadLocal));
   try {
@@ -395,7 +434,7 @@ int hd_batch_add(hd_batch *b, const uint8_t *packed, uint64_t n, uint64_t bytes_
     J->packed.alloc(std::max<uint64_t>(n * bytes_per_seq, 8));
     if (n)
       HD_HIP(hipMemcpyAsync(J->packed.get(), packed, n * bytes_per_seq, hipMemcpyHostToDevice, s));
-    if (semantics != HD_SEM_DENSE && n) {
+    if (n) { // the nibble mask picks the plane code
       DeviceBuffer d_mask(4);
       launch_nibble_mask(J->packed.get<uint8_t>(), n * bytes_per_seq, d_mask.get<uint32_t>(), s);
       HD_HIP(hipMemcpyAsync(&J->mask, d_mask.get(), 4, hipMemcpyDeviceToHost, s));

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -36,14 +37,26 @@ void ensure_device() {
 static constexpr uint32_t kLetters = (1u << 1) | (1u << 2) | (1u << 4) | (1u << 8);
 static constexpr uint32_t kGapBit = 1u << 15;
 static constexpr uint32_t kXBit = 1u << 3;
+static constexpr uint32_t kInvalidBit = 1u << 0;
 
-PlaneCode plane_code(int semantics, uint32_t mask) {
+PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
   PlaneCode pc;
   const uint8_t one_hot[4] = {1, 2, 4, 8};
   switch (semantics) {
   case HD_SEM_DENSE:
-    // The nibble itself: f = ((a & b) == 0) for every pair of codes.
     pc.planes = 4;
+    if ((mask & ~(kLetters | kGapBit | kInvalidBit) & 0xFFFFu) == 0 &&
+        32ull * words < (1ull << 30)) {
+      // Only 0, A, C, G, T and '-': the 3-op code, planes (g, c1, c2, c3).
+      pc.op = kDense3;
+      pc.table[1] = 0xE;  // A: c1 c2 c3
+      pc.table[2] = 0xC;  // C: c2 c3
+      pc.table[4] = 0xA;  // G: c1 c3
+      pc.table[8] = 0x8;  // T: c3
+      pc.table[15] = 0x3; // '-': g c1
+      return pc;
+    }
+    // The nibble itself: f = ((a & b) == 0) for every pair of codes.
     for (int v = 0; v < 16; ++v) pc.table[v] = (uint8_t)v;
     return pc;
   case HD_SEM_SPARSE:
@@ -57,6 +70,7 @@ PlaneCode plane_code(int semantics, uint32_t mask) {
     const bool has_x = x_class && (mask & kXBit);
     const bool has_inv = (mask & ~known & 0xFFFFu) != 0;
     pc.planes = 4 + (has_x ? 1 : 0) + (has_inv ? 1 : 0);
+    pc.op = pc.planes;
     const uint8_t all = (uint8_t)((1u << pc.planes) - 1);
     const uint8_t x_plane = has_x ? (uint8_t)(1u << 4) : 0;
     const uint8_t inv_plane = has_inv ? (uint8_t)(1u << (4 + (has_x ? 1 : 0))) : 0;
@@ -71,6 +85,7 @@ PlaneCode plane_code(int semantics, uint32_t mask) {
     // ones, every invalid code all zeros (distance 1 against anything).
     const bool has_x = (mask & kXBit) != 0;
     pc.planes = 4 + (has_x ? 1 : 0);
+    pc.op = pc.planes;
     for (int v = 0; v < 16; ++v) pc.table[v] = 0;
     for (int l = 0; l < 4; ++l) pc.table[one_hot[l]] = one_hot[l];
     pc.table[3] = has_x ? (uint8_t)(1u << 4) : 0;
@@ -110,15 +125,16 @@ static void launch_allpairs_t(const uint32_t *planes, uint32_t n,
                               uint64_t rb1, OutT *out, uint64_t cell_offset,
                               uint32_t max_dist, uint32_t check_every,
                               hipStream_t s) {
+  const uint32_t *corr =
+      P == kDense3 ? planes + (size_t)W * stored_planes(P) * n_pad : nullptr;
   constexpr uint64_t M = (64 * R) / (NW * C);
   const uint64_t t0 = tiles_before(rb0, M), t1 = tiles_before(rb1, M);
   const uint64_t nt = t1 - t0;
   if (nt == 0) return;
   if (nt > 0x7fffffffull) throw Error(HD_ERR_INVALID, "too many tiles in one launch");
-  (void)n_pad;
   hipLaunchKernelGGL((k_allpairs<P, R, C, NW, OutT>), dim3((uint32_t)nt),
                      dim3(64 * NW), 0, s, planes, n, W, t0, (uint32_t)nt,
-                     out, cell_offset, max_dist, check_every);
+                     out, cell_offset, max_dist, check_every, corr, n_pad);
   HD_HIP(hipGetLastError());
 }
 
@@ -152,6 +168,9 @@ static void launch_allpairs_o(int P, int v, const uint32_t *planes, uint32_t n,
                               uint32_t max_dist, uint32_t check_every,
                               hipStream_t s) {
   switch (P) {
+  case kDense3:
+    return launch_allpairs_p<kDense3, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
+                                            cell_offset, max_dist, check_every, s);
   case 4:
     return launch_allpairs_p<4, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
                                       cell_offset, max_dist, check_every, s);
@@ -177,7 +196,7 @@ Geometry geometry(uint64_t n, uint64_t bpr, int semantics, uint32_t mask) {
   g.bpr = bpr;
   g.n_pad = std::max<uint64_t>(kRowPad, (n + kRowPad - 1) / kRowPad * kRowPad);
   g.W = (uint32_t)((2 * bpr + 31) / 32);
-  g.code = plane_code(semantics, mask);
+  g.code = plane_code(semantics, mask, g.W);
   return g;
 }
 
@@ -194,7 +213,7 @@ void run_allpairs(const Geometry &g, const uint32_t *planes, int out_bytes,
                   uint64_t rb1, void *out, uint64_t cell_offset, hipStream_t s) {
   const uint32_t W = g.W;
   const uint32_t check_every = check_interval(early_exit, max_dist, W);
-  const int P = g.code.planes;
+  const int P = g.code.op;
   switch (out_bytes) {
   case 1:
     launch_allpairs_o<uint8_t>(P, variant, planes, (uint32_t)g.n,
@@ -233,18 +252,29 @@ static void check_max_dist(int out_bytes, uint32_t max_dist) {
 __global__ __launch_bounds__(256) void k_encode(const uint8_t *__restrict__ packed,
                                                 uint32_t n, uint64_t bpr,
                                                 uint32_t n_pad, uint32_t W,
-                                                int P, Table16 tbl, int identity,
+                                                int P, Table16 tbl, int mode,
                                                 uint32_t *__restrict__ planes) {
   __shared__ uint8_t s_tbl[16];
   if (threadIdx.x < 16) s_tbl[threadIdx.x] = tbl.v[threadIdx.x];
   __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_pad) return;
-  encode_run(packed, n, bpr, W, P, s_tbl, planes, i, blockIdx.y * kEncodeRun, identity != 0);
+  encode_run(packed, n, bpr, W, P, s_tbl, planes, i, blockIdx.y * kEncodeRun, mode);
+}
+
+// Z and G counts of the dense 3-op code, one workgroup per row group.
+__global__ __launch_bounds__(256) void k_dense3_counts(const uint32_t *__restrict__ planes,
+                                                       uint32_t W, uint32_t n_pad,
+                                                       uint32_t *__restrict__ corr) {
+  dense3_counts_group(planes, W, blockIdx.x, n_pad, corr);
+}
+
+int encode_mode(const PlaneCode &c) {
+  return is_dense3(c) ? kEncDense3 : is_identity(c) ? kEncIdentity : kEncTable;
 }
 
 bool is_identity(const PlaneCode &c) {
-  if (c.planes != 4) return false;
+  if (c.planes != 4 || c.op != 4) return false;
   for (int v = 0; v < 16; ++v)
     if (c.table[v] != v) return false;
   return true;
@@ -258,9 +288,14 @@ void launch_encode(const Geometry &g, const uint8_t *d_packed, uint32_t *d_plane
   HD_REQUIRE(runs <= 65535, "sequences too long for the encode grid");
   dim3 grid((uint32_t)((g.n_pad + 255) / 256), runs);
   hipLaunchKernelGGL(k_encode, grid, dim3(256), 0, s, d_packed, (uint32_t)g.n,
-                     g.bpr, (uint32_t)g.n_pad, g.W, g.code.planes, t,
-                     is_identity(g.code) ? 1 : 0, d_planes);
+                     g.bpr, (uint32_t)g.n_pad, g.W, g.code.planes, t, encode_mode(g.code),
+                     d_planes);
   HD_HIP(hipGetLastError());
+  if (is_dense3(g.code)) {
+    hipLaunchKernelGGL(k_dense3_counts, dim3((uint32_t)(g.n_pad / 64)), dim3(256), 0, s,
+                       d_planes, g.W, (uint32_t)g.n_pad, d_planes + g.plane_words());
+    HD_HIP(hipGetLastError());
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -348,35 +383,11 @@ __global__ __launch_bounds__(256) void k_letter_counts(const uint8_t *__restrict
 // ---------------------------------------------------------------------------
 // reference distances: one query against many packed rows
 // ---------------------------------------------------------------------------
-// One wavefront per row; lane l reads 8-byte units l, l + 64, ... of the row
-// (512 contiguous bytes per load instruction) and of the query (L2 resident),
-// then the wavefront sums its 64 partial counts.
-template <int SEM>
-__device__ __forceinline__ uint32_t site_count32(uint32_t q, uint32_t r) {
-  if constexpr (SEM == HD_SEM_DENSE) {
-    // (q & r) nibble == 0: smear each nibble's bits into its bit 3.
-    uint32_t c = q & r;
-    c |= c << 1;
-    c |= c << 2;
-    return __builtin_popcount(~c & 0x88888888u);
-  } else {
-    // (q & r) == 0 || (q != r && q != '-' && r != '-')
-    uint32_t z = q & r;
-    z |= z << 1;
-    z |= z << 2; // bit 3: q & r != 0
-    uint32_t d = q ^ r;
-    d |= d << 1;
-    d |= d << 2; // bit 3: q != r
-    uint32_t gq = q & (q << 1);
-    gq &= gq << 2; // bit 3: q == '-'
-    uint32_t gr = r & (r << 1);
-    gr &= gr << 2; // bit 3: r == '-'
-    const uint32_t f = ~z | (d & ~(gq | gr));
-    return __builtin_popcount(f & 0x88888888u);
-  }
-}
-
-template <int SEM>
+// A persistent grid of wavefronts: each holds the query in VGPRs (RegQuery)
+// and walks rows k = wave, wave + waves, ...; lane l reads 8-byte units
+// l, l + 64, ... of a row, all of them issued before the first is used.
+// Rows longer than the register form covers read the query from memory.
+template <int SEM, int T>
 __global__ __launch_bounds__(256) void k_refdist(const uint2 *__restrict__ query,
                                                  const uint8_t *__restrict__ db,
                                                  uint64_t bpr,
@@ -386,19 +397,54 @@ __global__ __launch_bounds__(256) void k_refdist(const uint2 *__restrict__ query
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t units = bpr / 8;
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
-  for (uint64_t k = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-       k < m; k += waves) {
-    const uint64_t r = rows ? rows[k] : k;
-    const uint2 *row = reinterpret_cast<const uint2 *>(db + r * bpr);
-    uint32_t cnt = 0;
-    for (uint64_t u = lane; u < units; u += 64) {
-      const uint2 a = row[u];
-      const uint2 q = query[u];
-      cnt += site_count32<SEM>(q.x, a.x) + site_count32<SEM>(q.y, a.y);
+  const uint64_t k0 = (uint64_t)blockIdx.x * (blockDim.x / 64) +
+                      __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  if constexpr (T > 0) {
+    RegQuery<SEM, T> q;
+    q.load(query, (uint32_t)units, lane);
+    for (uint64_t k = k0; k < m; k += waves) {
+      const uint64_t r = rows ? rows[k] : k;
+      const uint32_t d =
+          q.distance(reinterpret_cast<const uint2 *>(db + r * bpr), (uint32_t)units, lane);
+      if (lane == 0) out[k] = d;
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-    if (lane == 0) out[k] = cnt;
+  } else {
+    for (uint64_t k = k0; k < m; k += waves) {
+      const uint64_t r = rows ? rows[k] : k;
+      const uint32_t d = row_distance_mem<SEM>(
+          query, reinterpret_cast<const uint2 *>(db + r * bpr), units, lane);
+      if (lane == 0) out[k] = d;
+    }
+  }
+}
+
+// Workgroups of `block` threads of kernel `fn` resident on the whole current
+// device at once; looked up once per (device, kernel).
+static uint64_t resident_blocks(const void *fn, int block) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, uint64_t> cache;
+  int dev = 0;
+  HD_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(dev, fn);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int cus = 0, per_cu = 0;
+  HD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
+  const uint64_t r = (uint64_t)std::max(cus, 1) * std::max(per_cu, 1);
+  cache.emplace(key, r);
+  return r;
+}
+
+template <int SEM>
+static const void *refdist_kernel(int t) {
+  switch (t) {
+  case 4: return reinterpret_cast<const void *>(&k_refdist<SEM, 4>);
+  case 8: return reinterpret_cast<const void *>(&k_refdist<SEM, 8>);
+  case 16: return reinterpret_cast<const void *>(&k_refdist<SEM, 16>);
+  case 32: return reinterpret_cast<const void *>(&k_refdist<SEM, 32>);
+  default: return reinterpret_cast<const void *>(&k_refdist<SEM, 0>);
   }
 }
 
@@ -406,20 +452,19 @@ static void launch_refdist(const uint8_t *d_query, const uint8_t *d_db,
                            uint64_t bpr, const uint64_t *d_rows, uint64_t m,
                            int semantics, uint32_t *d_out, hipStream_t s) {
   if (m == 0) return;
-  const uint64_t wpb = 4; // wavefronts per block
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>((m + wpb - 1) / wpb, 1u << 16);
-  if (semantics == HD_SEM_DENSE)
-    hipLaunchKernelGGL(k_refdist<HD_SEM_DENSE>, dim3(blocks), dim3(64 * wpb), 0, s,
-                       reinterpret_cast<const uint2 *>(d_query), d_db, bpr, d_rows,
-                       m, d_out);
-  else if (semantics == HD_SEM_CHAR_X)
-    hipLaunchKernelGGL(k_refdist<HD_SEM_CHAR_X>, dim3(blocks), dim3(64 * wpb), 0, s,
-                       reinterpret_cast<const uint2 *>(d_query), d_db, bpr, d_rows,
-                       m, d_out);
-  else
+  if (semantics != HD_SEM_DENSE && semantics != HD_SEM_CHAR_X)
     throw Error(HD_ERR_INVALID,
                 "reference distances support HD_SEM_DENSE and HD_SEM_CHAR_X");
-  HD_HIP(hipGetLastError());
+  const int t = reg_query_t(bpr / 8);
+  const void *fn = semantics == HD_SEM_DENSE ? refdist_kernel<HD_SEM_DENSE>(t)
+                                             : refdist_kernel<HD_SEM_CHAR_X>(t);
+  // Workgroups of 4 wavefronts; as many as are resident at once (each
+  // wavefront loads the query once), but no more than the rows need.
+  const uint64_t resident = resident_blocks(fn, 256);
+  const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min(resident, (m + 3) / 4));
+  const uint2 *q = reinterpret_cast<const uint2 *>(d_query);
+  void *args[] = {&q, &d_db, &bpr, &d_rows, &m, &d_out};
+  HD_HIP(hipLaunchKernel(fn, dim3(blocks), dim3(256), args, 0, s));
 }
 
 // ---------------------------------------------------------------------------
@@ -549,8 +594,10 @@ static void upload_encode(Encoded &e, const uint8_t *packed, uint64_t n,
   geometry(n, bpr, HD_SEM_DENSE, 0); // argument checks
   e.packed.alloc(n * bpr);
   HD_HIP(hipMemcpyAsync(e.packed.get(), packed, n * bpr, hipMemcpyHostToDevice, s));
+  // The nibble mask picks the plane code (also for the dense semantics,
+  // whose 3-op code needs the data to hold no X or other codes).
   uint32_t mask = 0xFFFFu;
-  if (semantics != HD_SEM_DENSE) {
+  {
     e.mask.alloc(sizeof(uint32_t));
     launch_nibble_mask(e.packed.get<uint8_t>(), n * bpr, e.mask.get<uint32_t>(), s);
     HD_HIP(hipMemcpyAsync(&mask, e.mask.get(), 4, hipMemcpyDeviceToHost, s));

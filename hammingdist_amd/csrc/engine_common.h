@@ -131,17 +131,24 @@ private:
 };
 
 // Bit-sliced plane encoding of one dataset: every nibble code maps to a
-// P-bit mask, one bit per plane.  A site pair counts as distance 1 exactly
-// when no plane has the bit set in both sequences (sum of AND over planes is
-// zero), which is how every semantics of the engine is evaluated.
+// `planes`-bit mask, one bit per plane.  `op` names the tile kernel's pair
+// function (its template parameter P): for op = planes (4, 5, 6) a site pair
+// counts as distance 1 exactly when no plane has the bit set in both
+// sequences; op = 3 is the dense 3-op code over 4 planes whose rows also
+// carry per-row correction counts after the planes (see kernels.h).
 struct PlaneCode {
   int planes = 4;
+  int op = 4;
   uint8_t table[16] = {};
 };
 
-PlaneCode plane_code(int semantics, uint32_t nibble_mask);
+PlaneCode plane_code(int semantics, uint32_t nibble_mask, uint32_t words = 0);
 // True for the 4-plane code equal to the nibble (the dense semantics).
 bool is_identity(const PlaneCode &c);
+// True for the dense 3-op code.
+inline bool is_dense3(const PlaneCode &c) { return c.op == 3; }
+// encode_run mode (kernels.h EncodeMode) for a code.
+int encode_mode(const PlaneCode &c);
 
 inline uint64_t tri_cells(uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; }
 
@@ -150,7 +157,9 @@ struct Geometry {
   uint64_t n = 0, bpr = 0, n_pad = 0;
   uint32_t W = 0; // 32-site words per row
   PlaneCode code;
-  uint64_t total_words() const { return (uint64_t)W * code.planes * n_pad; }
+  uint64_t plane_words() const { return (uint64_t)W * code.planes * n_pad; }
+  // Planes, then (dense 3-op code) Z[n_pad] and G[n_pad].
+  uint64_t total_words() const { return plane_words() + (is_dense3(code) ? 2 * n_pad : 0); }
 };
 Geometry geometry(uint64_t n, uint64_t bpr, int semantics, uint32_t mask);
 

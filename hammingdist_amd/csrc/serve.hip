@@ -37,47 +37,32 @@ struct QueryDev {
 
 constexpr uint32_t kRowsPerBlock = 64; // 4 wavefronts x 16 rows
 
-template <int SEM>
-__device__ __forceinline__ uint32_t count_sites(uint32_t q, uint32_t r) {
-  if constexpr (SEM == HD_SEM_DENSE) {
-    // (q & r) nibble == 0, nibble bits smeared into bit 3
-    uint32_t c = q & r;
-    c |= c << 1;
-    c |= c << 2;
-    return __builtin_popcount(~c & 0x88888888u);
+template <int SEM, int T>
+__device__ __forceinline__ void query_rows(const QueryDev &Q, uint64_t k_begin, uint64_t k_end,
+                                           const uint8_t *__restrict__ db, uint64_t bpr,
+                                           uint32_t lane) {
+  const uint64_t units = bpr / 8;
+  if constexpr (T > 0) {
+    RegQuery<SEM, T> q;
+    q.load(Q.query, (uint32_t)units, lane);
+    for (uint64_t k = k_begin; k < k_end; k += 4) {
+      const uint32_t d = q.distance(reinterpret_cast<const uint2 *>(db + Q.rows[k] * bpr),
+                                    (uint32_t)units, lane);
+      if (lane == 0) Q.out[k] = d;
+    }
   } else {
-    // (q & r) == 0 || (q != r && neither is '-')
-    uint32_t z = q & r;
-    z |= z << 1;
-    z |= z << 2;
-    uint32_t d = q ^ r;
-    d |= d << 1;
-    d |= d << 2;
-    uint32_t gq = q & (q << 1);
-    gq &= gq << 2;
-    uint32_t gr = r & (r << 1);
-    gr &= gr << 2;
-    return __builtin_popcount((~z | (d & ~(gq | gr))) & 0x88888888u);
+    for (uint64_t k = k_begin; k < k_end; k += 4) {
+      const uint32_t d = row_distance_mem<SEM>(
+          Q.query, reinterpret_cast<const uint2 *>(db + Q.rows[k] * bpr), units, lane);
+      if (lane == 0) Q.out[k] = d;
+    }
   }
-}
-
-template <int SEM>
-__device__ __forceinline__ uint32_t row_distance(const uint2 *__restrict__ q,
-                                                 const uint2 *__restrict__ row, uint64_t units,
-                                                 uint32_t lane) {
-  uint32_t cnt = 0;
-  for (uint64_t u = lane; u < units; u += kWave) {
-    const uint2 a = row[u];
-    const uint2 b = q[u];
-    cnt += count_sites<SEM>(b.x, a.x) + count_sites<SEM>(b.y, a.y);
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-  return cnt;
 }
 
 // Many queries in one launch: block g belongs to the query whose block range
-// holds g and scans 64 of its rows, one wavefront per row at a time.
+// holds g and scans 64 of its rows; each wavefront loads the query once
+// (into VGPRs when T > 0) and scans 16 of those rows.
+template <int T>
 __global__ __launch_bounds__(256) void k_query_batch(const QueryDev *__restrict__ qs,
                                                      uint32_t n_q,
                                                      const uint8_t *__restrict__ db,
@@ -90,14 +75,22 @@ __global__ __launch_bounds__(256) void k_query_batch(const QueryDev *__restrict_
   }
   const QueryDev &Q = qs[__builtin_amdgcn_readfirstlane(lo)];
   const uint64_t r0 = (blockIdx.x - Q.block_begin) * kRowsPerBlock;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t units = bpr / 8;
-  for (uint64_t k = r0 + wave; k < r0 + kRowsPerBlock && k < Q.m; k += 4) {
-    const uint2 *row = reinterpret_cast<const uint2 *>(db + Q.rows[k] * bpr);
-    const uint32_t d = Q.semantics == HD_SEM_DENSE
-                           ? row_distance<HD_SEM_DENSE>(Q.query, row, units, lane)
-                           : row_distance<HD_SEM_CHAR_X>(Q.query, row, units, lane);
-    if (lane == 0) Q.out[k] = d;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t k_end = std::min<uint64_t>(r0 + kRowsPerBlock, Q.m);
+  if (Q.semantics == HD_SEM_DENSE)
+    query_rows<HD_SEM_DENSE, T>(Q, r0 + wave, k_end, db, bpr, lane);
+  else
+    query_rows<HD_SEM_CHAR_X, T>(Q, r0 + wave, k_end, db, bpr, lane);
+}
+
+static const void *query_batch_kernel(uint64_t bpr) {
+  switch (reg_query_t(bpr / 8)) {
+  case 4: return reinterpret_cast<const void *>(&k_query_batch<4>);
+  case 8: return reinterpret_cast<const void *>(&k_query_batch<8>);
+  case 16: return reinterpret_cast<const void *>(&k_query_batch<16>);
+  case 32: return reinterpret_cast<const void *>(&k_query_batch<32>);
+  default: return reinterpret_cast<const void *>(&k_query_batch<0>);
   }
 }
 
@@ -208,6 +201,7 @@ struct hd_server {
   };
 
   uint64_t n_db = 0, bpr = 0, max_batch = 64;
+  uint32_t db_mask = 0xFFFFu; // nibble codes present in the database
   int n_slots = 2;
   DeviceBuffer db, dev_mem;
   PinnedBuffer host_mem;
@@ -319,10 +313,12 @@ struct hd_server {
       HD_HIP(hipMemcpyAsync(dev_arena.at(b.table_d), host_arena.at(b.table_h), bytes,
                             hipMemcpyHostToDevice, s));
       HD_REQUIRE(blocks < (1ull << 31), "query batch too large");
-      hipLaunchKernelGGL(k_query_batch, dim3((uint32_t)blocks), dim3(256), 0, s,
-                         reinterpret_cast<const QueryDev *>(dev_arena.at(b.table_d)),
-                         (uint32_t)qd.size(), db.get<uint8_t>(), bpr);
-      HD_HIP(hipGetLastError());
+      const QueryDev *table = reinterpret_cast<const QueryDev *>(dev_arena.at(b.table_d));
+      uint32_t n_q = (uint32_t)qd.size();
+      const uint8_t *db_ptr = db.get<uint8_t>();
+      void *args[] = {&table, &n_q, &db_ptr, &bpr};
+      HD_HIP(hipLaunchKernel(query_batch_kernel(bpr), dim3((uint32_t)blocks), dim3(256), args,
+                             0, s));
     }
     for (uint64_t id : ids) {
       Request &r = *reqs[id];
@@ -456,6 +452,14 @@ int hd_server_create(const uint8_t *db, uint64_t n_db, uint64_t bytes_per_seq,
     s->max_batch = max_batch ? max_batch : 64;
     s->db.alloc(n_db * bytes_per_seq);
     HD_HIP(hipMemcpy(s->db.get(), db, n_db * bytes_per_seq, hipMemcpyHostToDevice));
+    {
+      // Codes present anywhere in the database: every all-pairs request on
+      // its rows uses the plane code this mask selects.
+      DeviceBuffer d_mask(4);
+      launch_nibble_mask(s->db.get<uint8_t>(), n_db * bytes_per_seq, d_mask.get<uint32_t>(),
+                         nullptr);
+      HD_HIP(hipMemcpy(&s->db_mask, d_mask.get(), 4, hipMemcpyDeviceToHost));
+    }
     s->dev_mem.alloc(device_bytes);
     s->host_mem.alloc(host_bytes);
     s->dev_arena.init(s->dev_mem.get<uint8_t>(), device_bytes);
@@ -483,7 +487,7 @@ static int submit(hd_server *s, std::unique_ptr<hd_server::Request> r, uint64_t 
   return guarded([&] {
     HD_REQUIRE(s && id, "null server or id");
     for (uint64_t row : r->rows) HD_REQUIRE(row < s->n_db, "row index out of range");
-    if (r->kind == hd_server::ALLPAIRS) r->g = geometry(r->m, s->bpr, r->semantics, 0xFFFFu);
+    if (r->kind == hd_server::ALLPAIRS) r->g = geometry(r->m, s->bpr, r->semantics, s->db_mask);
     r->lay = s->layout(*r);
     if (r->lay.dev > s->dev_arena.size() || r->lay.host > s->host_arena.size())
       throw Error(HD_ERR_CAPACITY, "request larger than the server's memory budget");

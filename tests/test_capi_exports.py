@@ -66,3 +66,10 @@ def test_invalid_arguments_are_rejected_before_device_use():
     assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, (1 << 3) | (1 << 0))[2] == 6
     assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, (1 << 1) | (1 << 15))[2] == 4
     assert capi.planes_layout(10, 8, capi.HD_SEM_CHAR_X, 1 << 3)[2] == 5
+    # dense data holding only A C G T '-' and invalid codes: the 3-op code,
+    # 4 planes plus two per-row counts; any other code (X = 3): nibble planes
+    acgt_gap_inv = (1 << 0) | (1 << 1) | (1 << 2) | (1 << 4) | (1 << 8) | (1 << 15)
+    n_pad, W, P, total = capi.planes_layout(1000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv)
+    assert (n_pad, W, P) == (1024, 935, 4) and total == 1024 * 935 * 4 + 2 * 1024
+    assert capi.planes_layout(1000, 14952, capi.HD_SEM_DENSE,
+                              acgt_gap_inv | (1 << 3))[3] == 1024 * 935 * 4

@@ -31,10 +31,12 @@ def run_dev(packed, sem, out_bytes, max_dist, early, variant):
 
 
 @pytest.mark.parametrize("variant", range(5))
-def test_every_variant_matches_oracle(variant, rng):
+@pytest.mark.parametrize("alphabet,include_x", [("ACGT-N", False), ("ACGTX-N", True)])
+def test_every_variant_matches_oracle(variant, alphabet, include_x, rng):
+    # ACGT-N: the dense 3-op code; X packed as code 3: the 4-plane code.
     for n, L in [(2, 3), (65, 64), (600, 300), (1100, 97)]:
-        seqs = random_strings(rng, n, L, "ACGT-N")
-        packed = O.pack(O.as_chars(seqs))
+        seqs = random_strings(rng, n, L, alphabet)
+        packed = O.pack(O.as_chars(seqs), include_x)
         for out_bytes, md in [(1, 255), (2, 7), (4, 2**32 - 1)]:
             want = O.all_pairs_dense(packed, out_bytes, md)
             for early in (False, True):

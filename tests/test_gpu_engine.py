@@ -30,17 +30,47 @@ def test_device_present():
     print("device arch:", capi.device_arch())
 
 
+# Dense inputs of three kinds: letters and '-' only, with invalid characters
+# (both take the 3-op code with per-row corrections), and packed with the
+# include_x table so that X (code 3) forces the 4-plane nibble code.
+DENSE_INPUTS = [("ACGT-", False), ("ACGT-N?", False), ("ACGTX-N", True)]
+
+
+@pytest.mark.parametrize("alphabet,include_x", DENSE_INPUTS)
 @pytest.mark.parametrize("n,L", [(2, 1), (3, 17), (7, 88), (11, 381), (127, 1023), (128, 33),
                                  (255, 64), (256, 95), (257, 1092), (513, 130), (703, 381),
                                  (1100, 700)])
-def test_dense_all_pairs_matches_oracle(n, L, rng):
-    seqs = random_strings(rng, n, L, "ACGT-")
-    packed = O.pack(O.as_chars(seqs))
+def test_dense_all_pairs_matches_oracle(n, L, alphabet, include_x, rng):
+    seqs = random_strings(rng, n, L, alphabet)
+    packed = O.pack(O.as_chars(seqs), include_x)
     for out_bytes, max_dist in [(1, 255), (2, 65535), (4, 2**32 - 1), (1, 3), (2, 89),
                                 (1, 0), (4, 497)]:
         want = O.all_pairs_dense(packed, out_bytes, max_dist)
         got = engine_all_pairs(packed, capi.HD_SEM_DENSE, out_bytes, max_dist)
         np.testing.assert_array_equal(got, want, err_msg=f"n={n} L={L} md={max_dist}")
+
+
+def test_dense_raw_nibbles_match_oracle(rng):
+    # Arbitrary nibble codes (not from the lookup table) take the 4-plane code.
+    n, bps = 300, 136
+    packed = rng.integers(0, 256, size=(n, bps), dtype=np.uint8)
+    for out_bytes, md in [(2, 65535), (1, 9)]:
+        want = O.all_pairs_dense(packed, out_bytes, md)
+        np.testing.assert_array_equal(
+            engine_all_pairs(packed, capi.HD_SEM_DENSE, out_bytes, md), want)
+
+
+@pytest.mark.parametrize("L", [1, 31, 32, 33, 100, 1025])
+def test_dense_invalid_and_gap_rows(L):
+    # Whole rows of invalid characters and of '-': the per-row corrections
+    # (invalid count of the row, '-' count of the column, padding included)
+    # at their extremes, around every clamp.
+    seqs = ["N" * L, "-" * L, "A" * L, "N" * L, "-" * L, "G" * L, "A" * (L - 1) + "N"]
+    packed = O.pack(O.as_chars(seqs))
+    for md in sorted({0, 1, max(L - 1, 0), L, L + 1, 2**31 - 1}):
+        want = O.all_pairs_dense(packed, 4, md)
+        got = engine_all_pairs(packed, capi.HD_SEM_DENSE, 4, md)
+        np.testing.assert_array_equal(got, want, err_msg=f"L={L} md={md}")
 
 
 @pytest.mark.parametrize("alphabet", ["ACGT-NX", "ACGT-", "AAAAAAAACGT-N", "ACGTXn*-?"])
@@ -138,6 +168,32 @@ def test_reference_distances_match_oracle(rng):
         out2 = np.zeros(sel.size, np.uint32)
         capi.reference_distances(q, db, m, db.shape[1], sel, sel.size, sem, out2)
         assert out2.tolist() == [want[int(k)] for k in sel]
+
+
+@pytest.mark.parametrize("units", [1, 63, 64, 65, 256, 257, 512, 513, 1024, 1025, 2048, 2049,
+                                   3001])
+def test_reference_distances_row_lengths(rng, units):
+    # Row lengths around the register-resident query sizes (64 T units of
+    # 8 bytes for T = 4, 8, 16, 32) and past them (query read from memory).
+    m, bps = 300, 8 * units
+    db = rng.integers(0, 256, size=(m, bps), dtype=np.uint8)
+    q = rng.integers(0, 256, size=bps, dtype=np.uint8)
+    db[::7] = q  # some exact matches (distance 0)
+    out = np.zeros(m, np.uint32)
+    capi.reference_distances(q, db, m, bps, None, m, capi.HD_SEM_DENSE, out)
+    assert out.tolist() == [O.distance_dense(q, db[k]) for k in range(m)]
+
+
+@pytest.mark.parametrize("L", [4095, 4096, 4097, 32767, 32768, 32769])
+def test_reference_distances_char_x_lengths(rng, L):
+    m = 70
+    seqs = random_strings(rng, m + 1, L, "ACGTXN-")
+    ref, rows = seqs[0], seqs[1:]
+    db = np.ascontiguousarray(O.pack(O.as_chars(rows), True))
+    q = np.ascontiguousarray(O.pack(O.as_chars([ref]), True)[0])
+    out = np.zeros(m, np.uint32)
+    capi.reference_distances(q, db, m, db.shape[1], None, m, capi.HD_SEM_CHAR_X, out)
+    assert out.tolist() == [O.distance_char(ref, s, True) for s in rows]
 
 
 def test_consensus_diff_matches_oracle(rng):
