@@ -33,7 +33,9 @@ sys.path.insert(0, ROOT)
 METRIC = "site-comparisons/s all-pairs n=50k L=29903; batched vs plain jobs/s; p99 ms"
 PEAK_INT32_OPS = 256 * 64 * 2.4e9  # CUs x lanes x clock: INT32 VALU lane-ops/s
 PEAK_HBM = 8.0e12
-OPS_PER_WORD = {4: 5, 5: 6, 6: 7}  # VALU ops per pair per 32-site word, by plane count
+# VALU ops per pair per 32-site word, by stored plane count: P + 1 for the
+# one-hot and class codes
+OPS_PER_WORD = {3: 4, 4: 5, 5: 6}
 OPS_PER_WORD_DENSE3 = 4  # the dense 3-op code: three v_bitop3 + popcount-accumulate
 
 

@@ -116,8 +116,8 @@ __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_allpairs(
     uint32_t b, cb;
     tile_coords(t - J.tile_begin, TI / TJ, b, cb);
     const uint32_t i0 = b * TI, j0 = cb * TJ + wave * C;
-    // 5 and 6 planes with 16 columns per wave would need more column words
-    // than the SGPR file holds: such tiles run as two passes of 8 columns.
+    // 5 planes with 16 columns per wave would need more column words than
+    // the SGPR file holds: such tiles run as two passes of 8 columns.
     constexpr int CW = C > 8 ? 8 : C;
 #define HD_BATCH_TILE(PP, CC, JJ)                                                        \
   tile_wave<PP, R, CC>(J.planes, J.n, J.W, i0, JJ, J.out, J.out_bytes, 0, J.max_dist,     \
@@ -129,14 +129,15 @@ __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_allpairs(
     case 4:
       HD_BATCH_TILE(4, C, j0)
       break;
-    case 5:
-      for (uint32_t h = 0; h < (uint32_t)(C / CW); ++h) {
-        HD_BATCH_TILE(5, CW, j0 + h * CW)
-      }
+    case kClass2:
+      HD_BATCH_TILE(kClass2, C, j0)
+      break;
+    case kClass3:
+      HD_BATCH_TILE(kClass3, C, j0)
       break;
     default:
       for (uint32_t h = 0; h < (uint32_t)(C / CW); ++h) {
-        HD_BATCH_TILE(6, CW, j0 + h * CW)
+        HD_BATCH_TILE(5, CW, j0 + h * CW)
       }
       break;
     }
@@ -166,13 +167,15 @@ __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_grid(
   } else if (P == 4) {
     tile_wave<4, R, C>(J.planes, J.n, J.W, i0, j0, J.out, J.out_bytes, 0, J.max_dist,
                        J.check_every, nullptr, J.n_pad);
-  } else if (P == 5) {
-    for (uint32_t h = 0; h < (uint32_t)(C / CW); ++h)
-      tile_wave<5, R, CW>(J.planes, J.n, J.W, i0, j0 + h * CW, J.out, J.out_bytes, 0,
-                          J.max_dist, J.check_every, nullptr, J.n_pad);
+  } else if (P == kClass2) {
+    tile_wave<kClass2, R, C>(J.planes, J.n, J.W, i0, j0, J.out, J.out_bytes, 0, J.max_dist,
+                             J.check_every, nullptr, J.n_pad);
+  } else if (P == kClass3) {
+    tile_wave<kClass3, R, C>(J.planes, J.n, J.W, i0, j0, J.out, J.out_bytes, 0, J.max_dist,
+                             J.check_every, nullptr, J.n_pad);
   } else {
     for (uint32_t h = 0; h < (uint32_t)(C / CW); ++h)
-      tile_wave<6, R, CW>(J.planes, J.n, J.W, i0, j0 + h * CW, J.out, J.out_bytes, 0,
+      tile_wave<5, R, CW>(J.planes, J.n, J.W, i0, j0 + h * CW, J.out, J.out_bytes, 0,
                           J.max_dist, J.check_every, nullptr, J.n_pad);
   }
 }

@@ -46,7 +46,7 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
   case HD_SEM_DENSE:
     pc.planes = 4;
     if ((mask & ~(kLetters | kGapBit | kInvalidBit) & 0xFFFFu) == 0 &&
-        32ull * words < (1ull << 30)) {
+        32ull * words < (1ull << 28)) {
       // Only 0, A, C, G, T and '-': the 3-op code, planes (g, c1, c2, c3).
       pc.op = kDense3;
       pc.table[1] = 0xE;  // A: c1 c2 c3
@@ -61,23 +61,28 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
     return pc;
   case HD_SEM_SPARSE:
   case HD_SEM_SPARSE_X: {
-    // f = codes differ and neither is '-': one-hot classes, '-' all ones.
-    // X is a class of its own only for SPARSE_X; every code that is not a
-    // letter, '-' or (SPARSE_X) X is the invalid class.  Classes absent from
-    // the data need no plane.
+    // f = classes differ and neither is '-'.  Classes: the letters, X
+    // (SPARSE_X only) and one invalid class for every other code but '-'.
+    // Each class present gets a code 0, 1, ...; 2 code bits for up to 4
+    // classes, else 3; '-' sets the plane after the code bits.
     const bool x_class = semantics == HD_SEM_SPARSE_X;
     const uint32_t known = kLetters | kGapBit | (x_class ? kXBit : 0u);
-    const bool has_x = x_class && (mask & kXBit);
-    const bool has_inv = (mask & ~known & 0xFFFFu) != 0;
-    pc.planes = 4 + (has_x ? 1 : 0) + (has_inv ? 1 : 0);
-    pc.op = pc.planes;
-    const uint8_t all = (uint8_t)((1u << pc.planes) - 1);
-    const uint8_t x_plane = has_x ? (uint8_t)(1u << 4) : 0;
-    const uint8_t inv_plane = has_inv ? (uint8_t)(1u << (4 + (has_x ? 1 : 0))) : 0;
-    for (int v = 0; v < 16; ++v) pc.table[v] = inv_plane;
-    for (int l = 0; l < 4; ++l) pc.table[one_hot[l]] = one_hot[l];
-    if (x_class) pc.table[3] = x_plane;
-    pc.table[15] = all;
+    int next = 0;
+    uint8_t code[16];
+    int inv_code = -1;
+    if ((mask & ~known & 0xFFFFu) != 0) inv_code = next++;
+    for (int l = 0; l < 4; ++l)
+      code[one_hot[l]] = (mask >> one_hot[l]) & 1u ? (uint8_t)next++ : 0;
+    const int x_code = x_class && (mask & kXBit) ? next++ : 0;
+    const int bits = next <= 4 ? 2 : 3;
+    pc.op = bits == 2 ? kClass2 : kClass3;
+    pc.planes = bits + 1;
+    for (int v = 0; v < 16; ++v) {
+      if (v == 15) pc.table[v] = (uint8_t)(1u << bits);
+      else if ((kLetters >> v) & 1u) pc.table[v] = code[v];
+      else if (x_class && v == 3) pc.table[v] = (uint8_t)x_code;
+      else pc.table[v] = (uint8_t)(inv_code < 0 ? 0 : inv_code);
+    }
     return pc;
   }
   case HD_SEM_CHAR_X: {
@@ -104,7 +109,7 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
 static const Variant kVariants[] = {
     {4, 16, 4}, // 256 x 64 tiles: the large-n shape for 4 planes
     {8, 8, 4},  // 512 x 32
-    {4, 8, 4},  // 256 x 32: 5-6 planes (fewer SGPRs per column block)
+    {4, 8, 4},  // 256 x 32: 5 planes (fewer SGPRs per column block)
     {2, 16, 4}, // 128 x 64
     {2, 8, 4},  // 128 x 32: small datasets
 };
@@ -115,7 +120,7 @@ const Variant &variant_of(int v) { return kVariants[v]; }
 
 int default_variant(int planes, uint64_t n) {
   if (n < 512) return 4;
-  if (planes == 4) return n < 2048 ? 3 : 0;
+  if (planes <= 4) return n < 2048 ? 3 : 0;
   return 2;
 }
 
@@ -132,9 +137,14 @@ static void launch_allpairs_t(const uint32_t *planes, uint32_t n,
   const uint64_t nt = t1 - t0;
   if (nt == 0) return;
   if (nt > 0x7fffffffull) throw Error(HD_ERR_INVALID, "too many tiles in one launch");
-  hipLaunchKernelGGL((k_allpairs<P, R, C, NW, OutT>), dim3((uint32_t)nt),
-                     dim3(64 * NW), 0, s, planes, n, W, t0, (uint32_t)nt,
-                     out, cell_offset, max_dist, check_every, corr, n_pad);
+  if (check_every < W)
+    hipLaunchKernelGGL((k_allpairs<P, R, C, NW, OutT, true>), dim3((uint32_t)nt),
+                       dim3(64 * NW), 0, s, planes, n, W, t0, (uint32_t)nt,
+                       out, cell_offset, max_dist, check_every, corr, n_pad);
+  else
+    hipLaunchKernelGGL((k_allpairs<P, R, C, NW, OutT, false>), dim3((uint32_t)nt),
+                       dim3(64 * NW), 0, s, planes, n, W, t0, (uint32_t)nt,
+                       out, cell_offset, max_dist, check_every, corr, n_pad);
   HD_HIP(hipGetLastError());
 }
 
@@ -177,9 +187,12 @@ static void launch_allpairs_o(int P, int v, const uint32_t *planes, uint32_t n,
   case 5:
     return launch_allpairs_p<5, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
                                       cell_offset, max_dist, check_every, s);
-  case 6:
-    return launch_allpairs_p<6, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
-                                      cell_offset, max_dist, check_every, s);
+  case kClass2:
+    return launch_allpairs_p<kClass2, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
+                                            cell_offset, max_dist, check_every, s);
+  case kClass3:
+    return launch_allpairs_p<kClass3, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
+                                            cell_offset, max_dist, check_every, s);
   default:
     throw Error(HD_ERR_INVALID, "unsupported plane count");
   }
@@ -202,10 +215,12 @@ Geometry geometry(uint64_t n, uint64_t bpr, int semantics, uint32_t mask) {
 
 static uint32_t tile_rows(int v) { return 64u * kVariants[v].R; }
 
+static_assert(kCheckWords == 32, "Geometry::total_words assumes 32-word chunks");
+
 uint32_t check_interval(bool early_exit, uint32_t max_dist, uint32_t W) {
-  // The early-exit vote runs every 32 words (1024 sites); it is skipped when
-  // max_dist is out of reach.
-  return (early_exit && (uint64_t)max_dist < 32ull * W) ? 32u : W;
+  // The early-exit vote runs every kCheckWords words (1024 sites); it is
+  // skipped when max_dist is out of reach.
+  return (early_exit && (uint64_t)max_dist < 32ull * W) ? kCheckWords : W;
 }
 
 void run_allpairs(const Geometry &g, const uint32_t *planes, int out_bytes,

@@ -52,14 +52,37 @@ struct Table16 {
 //                  s1 = g_i | (c1_i ^ c1_j)
 //                  s2 = c2_i ? (c2_j & ~s1) : ~(c2_j | s1)
 //                  s3 = c3_i ? ~(c3_j & s2) : ~(c3_j | s2)
-//                popcount(s3) + Z_i - G_j is the distance, where Z_i counts
-//                the invalid sites of row i and G_j the '-' sites of row j
-//                (padding included); both come from dense3_counts.
+//                Per site, f3 + [site of row i invalid] - [site of row j
+//                '-'] is the site's distance term, so popcount(s3) + Z_i - G_j
+//                is the distance, where Z_i counts the invalid sites of row i
+//                and G_j the '-' sites of row j (padding included).  With the
+//                early exit on, the counts of each kCheckWords-word chunk are
+//                added after that chunk, so every vote sees the exact partial
+//                distance; both come from dense3_counts.
+//   P = kClass2, kClass3  a site counts when the classes differ and neither
+//                is '-' (the sparse semantics).  Every class present gets a
+//                2- or 3-bit code; '-' sets the last plane.  Per pair and 32
+//                sites: one XOR, one (x ^ y) | s per further code bit, one
+//                s & ~g_i & ~g_j and one popcount-accumulate (4 or 5 ops).
 // ---------------------------------------------------------------------------
 constexpr int kDense3 = 3;
+constexpr int kClass2 = 7;
+constexpr int kClass3 = 8;
+
+// Words between two early-exit votes (1024 sites).
+constexpr uint32_t kCheckWords = 32;
+
+// kDense3 correction counts after the planes, in words from the first:
+// Z[n_pad], G[n_pad] (whole rows), then Zc[T][n_pad], Gc[T][n_pad] per chunk
+// of kCheckWords words, T = ceil(W / kCheckWords).
+__host__ __device__ constexpr uint64_t dense3_corr_words(uint32_t W, uint64_t n_pad) {
+  return (2 + 2 * (uint64_t)((W + kCheckWords - 1) / kCheckWords)) * n_pad;
+}
 
 // Planes stored per row and word for plane code P.
-__host__ __device__ constexpr int stored_planes(int P) { return P == kDense3 ? 4 : P; }
+__host__ __device__ constexpr int stored_planes(int P) {
+  return P == kDense3 ? 4 : P == kClass2 ? 3 : P == kClass3 ? 4 : P;
+}
 
 // Tile t -> (row block b, column block cb) of a triangle tiled with M column
 // blocks per row block: row block b holds tiles [M b(b+1)/2, M (b+1)(b+2)/2).
@@ -75,8 +98,10 @@ __device__ __forceinline__ void tile_coords(uint64_t t, uint32_t M, uint32_t &b,
 // One wave's share of a tile: rows i0 + 64 q + lane (q < R) against columns
 // j0 .. j0 + C - 1, stored into the triangle at cell i(i-1)/2 + j - cell_offset
 // as out_bytes-wide cells (1, 2 or 4; a constant where the caller's is).
-// corr (kDense3 only): Z[n_pad] then G[n_pad] of the planes' rows.
-template <int P, int R, int C>
+// corr (kDense3 only): the correction counts of the planes' rows
+// (dense3_corr_words); check_every is W or kCheckWords.  EE = false
+// compiles the loop without early-exit votes (check_every is then ignored).
+template <int P, int R, int C, bool EE = true>
 __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, uint32_t n,
                                           uint32_t W, uint32_t i0, uint32_t j0,
                                           void *__restrict__ out, uint32_t out_bytes,
@@ -85,14 +110,18 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
                                           const uint32_t *__restrict__ corr,
                                           uint32_t n_pad) {
   constexpr bool D3 = P == kDense3;
+  constexpr bool CLS = P == kClass2 || P == kClass3;
   constexpr int PS = stored_planes(P); // planes per row and word
-  constexpr int PB = D3 ? 3 : P;       // column planes used
+  constexpr int PB = D3 ? 3 : PS;      // column planes used
   constexpr int B0 = D3 ? 1 : 0;       // first column plane used
-  // kDense3 accumulators carry OFF + Z_i - G_j + popcounts (never below OFF
-  // at the end); distances never exceed 2^30 sites, so clamping the cap
-  // there keeps OFF + md in range.
+  // kDense3 accumulators carry OFF + popcounts + Z - G counts added so far.
+  // Rows have fewer than 2^28 sites (the code is only chosen then), so
+  // every partial sum stays within [OFF - 2^28, OFF + 2^29], the cap can be
+  // clamped to 2^28 and pairs outside the triangle start at kD3Idle, which
+  // stays above OFF + md and below 2^32.
   constexpr uint32_t OFF = D3 ? 0x80000000u : 0u;
-  const uint32_t md = D3 ? min(max_dist, 0x40000000u) : max_dist;
+  constexpr uint32_t kD3Idle = 0xD0000000u;
+  const uint32_t md = D3 ? min(max_dist, 0x10000000u) : max_dist;
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t i_end = min(i0 + (uint32_t)(kWave * R), n);
   if (j0 + 1 >= i_end) return; // no pair j < i inside this wave's block
@@ -100,17 +129,40 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   // Pairs outside the triangle start saturated so that the early-exit vote
   // only waits for real pairs; they are never stored.
   uint32_t acc[R][C];
-  if constexpr (D3) {
+  // Adds row counts zc[row] - gc[column] to every accumulator (kDense3).
+  // The column counts are read into VGPRs (the offset `vzero` is an opaque
+  // per-lane 0), which keeps them out of the SGPR file the column planes
+  // fill.
+  auto add_counts = [&](const uint32_t *__restrict__ zc, const uint32_t *__restrict__ gc) {
+    uint32_t vzero;
+    __asm__ volatile("v_mov_b32 %0, 0" : "=v"(vzero));
+    uint32_t gj[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) gj[c] = gc[j0 + c + vzero];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const uint32_t zi = zc[min(i0 + kWave * q + lane, n_pad - 1)];
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc[q][c] += zi - gj[c];
+    }
+  };
+  if constexpr (D3 && EE) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const uint32_t i = i0 + kWave * q + lane;
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc[q][c] = (j0 + c < i && i < n) ? OFF : kD3Idle;
+    }
+  } else if constexpr (D3) {
+    // No votes: the whole-row counts go in at the start.
     uint32_t gj[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) gj[c] = corr[n_pad + j0 + c];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const uint32_t i = i0 + kWave * q + lane;
-      const uint32_t zi = i < n ? corr[i] : 0u;
+      const uint32_t zi = corr[min(i0 + kWave * q + lane, n_pad - 1)];
 #pragma unroll
-      for (int c = 0; c < C; ++c)
-        acc[q][c] = (j0 + c < i && i < n) ? OFF + zi - gj[c] : OFF + md;
+      for (int c = 0; c < C; ++c) acc[q][c] = OFF + zi - gj[c];
     }
   } else {
 #pragma unroll
@@ -130,9 +182,9 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   using cptr = const __attribute__((address_space(4))) uint32_t *;
   cptr pa = (cptr)(planes + (size_t)(i0 / kWave) * group + lane);
   cptr pb = (cptr)(planes + (size_t)(j0 / kWave) * group + (j0 % kWave));
-  uint32_t k = 0;
+  uint32_t k = 0, chunk = 0;
   while (k < W) {
-    const uint32_t k_end = min(W, k + check_every);
+    const uint32_t k_end = EE ? min(W, k + check_every) : W;
     for (; k < k_end; ++k) {
       uint32_t a[R][PS];
 #pragma unroll
@@ -155,6 +207,14 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
             s = __builtin_amdgcn_bitop3_b32(a[q][1], bw[c][0], a[q][0], 0xBE); // z|(x^y)
             s = __builtin_amdgcn_bitop3_b32(a[q][2], bw[c][1], s, 0x41);
             s = __builtin_amdgcn_bitop3_b32(a[q][3], bw[c][2], s, 0x71);
+          } else if constexpr (CLS) {
+            // code bits differ (0xBE = (x^y)|z), then z & ~x & ~y (0x02)
+            // with the two '-' planes
+            s = a[q][0] ^ bw[c][0];
+#pragma unroll
+            for (int p = 1; p < PS - 1; ++p)
+              s = __builtin_amdgcn_bitop3_b32(a[q][p], bw[c][p], s, 0xBE);
+            s = __builtin_amdgcn_bitop3_b32(a[q][PS - 1], bw[c][PS - 1], s, 0x02);
           } else {
             // s = a0&b0; s = (ap&bp)|s ...; s = ~((a_last&b_last)|s):
             // 0xEA = (x&y)|z, 0x15 = its NOT.
@@ -169,13 +229,30 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
       pa += PS * kWave;
       pb += PS * kWave;
     }
-    if (check_every < W) {
+    if constexpr (D3 && EE) {
+      if (check_every < W) {
+        // The barrier keeps the count loads here, after the chunk, instead
+        // of holding them in registers across it.
+        __asm__ volatile("" ::: "memory");
+        const uint64_t T = (W + kCheckWords - 1) / kCheckWords;
+        add_counts(corr + (2 + chunk) * (uint64_t)n_pad,
+                   corr + (2 + T + chunk) * (uint64_t)n_pad);
+      }
+    }
+    ++chunk;
+    if (EE && check_every < W) {
       uint32_t lo = acc[0][0];
 #pragma unroll
       for (int q = 0; q < R; ++q)
 #pragma unroll
         for (int c = 0; c < C; ++c) lo = min(lo, acc[q][c]);
       if (__all(lo >= OFF + md)) break;
+    }
+  }
+  if constexpr (D3 && EE) {
+    if (check_every >= W) { // one chunk: the whole-row counts
+      __asm__ volatile("" ::: "memory");
+      add_counts(corr, corr + n_pad);
     }
   }
 
@@ -197,7 +274,7 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   }
 }
 
-template <int P, int R, int C, int NW, typename OutT>
+template <int P, int R, int C, int NW, typename OutT, bool EE>
 __global__ __launch_bounds__(kWave *NW) void k_allpairs(
     const uint32_t *__restrict__ planes, uint32_t n, uint32_t W, uint64_t tile_begin,
     uint32_t n_tiles, OutT *__restrict__ out, uint64_t cell_offset, uint32_t max_dist,
@@ -215,8 +292,8 @@ __global__ __launch_bounds__(kWave *NW) void k_allpairs(
   uint32_t b, cb;
   tile_coords(t, TI / TJ, b, cb);
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  tile_wave<P, R, C>(planes, n, W, b * TI, cb * TJ + wave * C, out, sizeof(OutT), cell_offset,
-                     max_dist, check_every, corr, n_pad);
+  tile_wave<P, R, C, EE>(planes, n, W, b * TI, cb * TJ + wave * C, out, sizeof(OutT),
+                         cell_offset, max_dist, check_every, corr, n_pad);
 }
 
 // ---------------------------------------------------------------------------
@@ -324,28 +401,38 @@ __device__ __forceinline__ void encode_run(const uint8_t *__restrict__ packed, u
 }
 
 // Per-row counts of the kDense3 code for row group g (64 rows): Z = invalid
-// sites (all four planes clear), G = '-' sites (plane 0), over all 32 W
-// sites of the row, written to corr[row] and corr[n_pad + row].  One
-// 256-thread workgroup per group: wave w sums the words k = w mod 4 for the
-// 64 rows (lane = row, 256-byte loads), then wave 0 adds the four partials.
+// sites (all four planes clear), G = '-' sites (plane 0), per chunk of
+// kCheckWords words and over the whole row (layout: dense3_corr_words).  One
+// 256-thread workgroup per group: wave w counts the chunks t = w mod 4 for
+// the 64 rows (lane = row, 256-byte loads), then wave 0 adds the four
+// partial row totals.
 __device__ __forceinline__ void dense3_counts_group(const uint32_t *__restrict__ planes,
                                                     uint32_t W, uint32_t g, uint32_t n_pad,
                                                     uint32_t *__restrict__ corr) {
   __shared__ uint32_t s_z[4][kWave], s_g[4][kWave];
   const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const uint32_t row = g * kWave + lane;
+  const uint32_t T = (W + kCheckWords - 1) / kCheckWords;
   const uint32_t *base = planes + (size_t)g * W * 4 * kWave + lane;
   uint32_t z = 0, gc = 0;
-  for (uint32_t k = wave; k < W; k += 4) {
-    const uint32_t *w = base + (size_t)k * 4 * kWave;
-    const uint32_t p0 = w[0], p1 = w[kWave], p2 = w[2 * kWave], p3 = w[3 * kWave];
-    z += __builtin_popcount(~(p0 | p1 | p2 | p3));
-    gc += __builtin_popcount(p0);
+  for (uint32_t t = wave; t < T; t += 4) {
+    uint32_t zt = 0, gt = 0;
+    const uint32_t k_end = min(W, (t + 1) * kCheckWords);
+    for (uint32_t k = t * kCheckWords; k < k_end; ++k) {
+      const uint32_t *w = base + (size_t)k * 4 * kWave;
+      const uint32_t p0 = w[0], p1 = w[kWave], p2 = w[2 * kWave], p3 = w[3 * kWave];
+      zt += __builtin_popcount(~(p0 | p1 | p2 | p3));
+      gt += __builtin_popcount(p0);
+    }
+    corr[(2 + (uint64_t)t) * n_pad + row] = zt;
+    corr[(2 + (uint64_t)T + t) * n_pad + row] = gt;
+    z += zt;
+    gc += gt;
   }
   s_z[wave][lane] = z;
   s_g[wave][lane] = gc;
   __syncthreads();
   if (wave == 0) {
-    const uint32_t row = g * kWave + lane;
     corr[row] = s_z[0][lane] + s_z[1][lane] + s_z[2][lane] + s_z[3][lane];
     corr[n_pad + row] = s_g[0][lane] + s_g[1][lane] + s_g[2][lane] + s_g[3][lane];
   }

@@ -63,13 +63,20 @@ def test_invalid_arguments_are_rejected_before_device_use():
     # planes layout is pure host arithmetic
     n_pad, W, P, total = capi.planes_layout(1000, 14952, capi.HD_SEM_DENSE, 0xFFFF)
     assert (n_pad, W, P) == (1024, 935, 4) and total == 1024 * 935 * 4
-    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, (1 << 3) | (1 << 0))[2] == 6
-    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, (1 << 1) | (1 << 15))[2] == 4
+    # sparse semantics: a 2-bit class code for up to 4 classes present, else
+    # 3 bits, plus the '-' plane
+    letters_gap = (1 << 1) | (1 << 2) | (1 << 4) | (1 << 8) | (1 << 15)
+    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE, letters_gap)[2] == 3
+    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE, letters_gap | 1)[2] == 4
+    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, 0xFFFF)[2] == 4
+    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, (1 << 3) | (1 << 0))[2] == 3
+    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, (1 << 1) | (1 << 15))[2] == 3
     assert capi.planes_layout(10, 8, capi.HD_SEM_CHAR_X, 1 << 3)[2] == 5
     # dense data holding only A C G T '-' and invalid codes: the 3-op code,
-    # 4 planes plus two per-row counts; any other code (X = 3): nibble planes
+    # 4 planes plus per-row counts; any other code (X = 3): nibble planes
     acgt_gap_inv = (1 << 0) | (1 << 1) | (1 << 2) | (1 << 4) | (1 << 8) | (1 << 15)
     n_pad, W, P, total = capi.planes_layout(1000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv)
-    assert (n_pad, W, P) == (1024, 935, 4) and total == 1024 * 935 * 4 + 2 * 1024
+    # (whole-row counts Z, G and per 32-word chunk counts: 2 + 2 * 30 rows)
+    assert (n_pad, W, P) == (1024, 935, 4) and total == 1024 * 935 * 4 + 62 * 1024
     assert capi.planes_layout(1000, 14952, capi.HD_SEM_DENSE,
                               acgt_gap_inv | (1 << 3))[3] == 1024 * 935 * 4

@@ -73,6 +73,28 @@ def test_dense_invalid_and_gap_rows(L):
         np.testing.assert_array_equal(got, want, err_msg=f"L={L} md={md}")
 
 
+def test_dense_early_exit_with_late_gaps_and_invalid(rng):
+    # Several 1024-site chunks; gap runs and invalid runs at varying places,
+    # few substitutions: distances straddle small caps, so the early-exit
+    # votes (taken on exact partial distances) decide for some tiles only.
+    n, L = 400, 6000
+    root = rng.choice(list("ACGT"), size=L)
+    seqs = []
+    for _ in range(n):
+        s = root.copy()
+        s[rng.integers(0, L, size=rng.integers(0, 6))] = rng.choice(list("ACGT"))
+        a = int(rng.integers(0, L))
+        s[a:a + int(rng.integers(0, 900))] = "-"
+        b = int(rng.integers(0, L))
+        s[b:b + int(rng.integers(0, 300))] = "N"
+        seqs.append("".join(s))
+    packed = O.pack(O.as_chars(seqs))
+    for out_bytes, md in [(1, 0), (1, 1), (1, 2), (1, 5), (2, 300), (2, 1200)]:
+        want = O.all_pairs_dense(packed, out_bytes, md)
+        got = engine_all_pairs(packed, capi.HD_SEM_DENSE, out_bytes, md)
+        np.testing.assert_array_equal(got, want, err_msg=f"md={md}")
+
+
 @pytest.mark.parametrize("alphabet", ["ACGT-NX", "ACGT-", "AAAAAAAACGT-N", "ACGTXn*-?"])
 @pytest.mark.parametrize("n,L", [(50, 25), (300, 513), (700, 97)])
 def test_sparse_semantics_match_oracle(alphabet, n, L, rng):
