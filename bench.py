@@ -24,6 +24,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -31,7 +32,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "site-comparisons/s all-pairs n=50k L=29903; batched vs plain jobs/s; p99 ms"
-PEAK_INT32_OPS = 256 * 64 * 2.4e9  # CUs x lanes x clock: INT32 VALU lane-ops/s
+# Nominal INT32 VALU rate from MI355X's published CU count and peak clock:
+# 256 CUs x 64 lanes x 2.4 GHz lane-ops/s.  The issue rate measured by
+# hd_valu_probe on the same instruction mix is reported beside it.
+PEAK_INT32_OPS = 256 * 64 * 2.4e9
 PEAK_HBM = 8.0e12
 # VALU ops per pair per 32-site word, by stored plane count: P + 1 for the
 # one-hot and class codes
@@ -155,6 +159,64 @@ class Timer:
         return a.elapsed_time(b) / reps
 
 
+def run_sparse_emit(torch, capi, np, d_cells, host_head, n, md, sh):
+    """(i, j, d) triplets of the resident uint8 triangle with d <= threshold
+    (DataSet::dump_sparse's selection, on the device): threshold 3 as the
+    large workload asks (every cell, since they are clamped at max_distance
+    2) and threshold 1 (near-duplicate pairs).  HBM-bound: the cells are
+    read once and the triplets written once (algorithmic bytes)."""
+    out = {}
+    cells = n * (n - 1) // 2
+    for thr in (3, 1):
+        count = capi.sparse_emit_dev(d_cells, n, 1, thr, None, 0, sh)
+        d_trip = torch.empty(max(count, 1) * 3, dtype=torch.int32, device="cuda")
+        capi.sparse_emit_dev(d_cells, n, 1, thr, d_trip, count, sh)  # warm-up
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(torch.cuda.current_stream())
+        got = capi.sparse_emit_dev(d_cells, n, 1, thr, d_trip, count, sh)
+        t1.record(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1)
+        assert got == count
+        # the first triplets against the cells already checked on the host
+        k = min(count, 20000)
+        trip = d_trip[: 3 * k].cpu().numpy().view(np.uint32).reshape(-1, 3)
+        I = trip[:, 0].astype(np.int64)
+        J = trip[:, 1].astype(np.int64)
+        x = I * (I - 1) // 2 + J
+        ok = x < host_head.size
+        assert (np.diff(x) > 0).all(), "triplets out of row-major order"
+        assert (trip[ok, 2] == host_head[x[ok]]).all()
+        assert (host_head[x[ok]] <= thr).all()
+        if x[ok].size and thr >= md:
+            assert x[ok][-1] == x[ok].size - 1, "every cell is at most max_distance"
+        alg = cells + 12 * count
+        out[f"threshold_{thr}"] = {"triplets": count, "ms": ms,
+                                   "GB_per_s": alg / (ms / 1e3) / 1e9,
+                                   "hbm_frac": alg / (ms / 1e3) / PEAK_HBM}
+        log(f"sparse emit threshold {thr}: {count} triplets in {ms:.2f} ms, "
+            f"{alg / (ms / 1e3) / 1e9:.0f} GB/s")
+        del d_trip
+        torch.cuda.empty_cache()
+    return out
+
+
+def valu_ceiling():
+    """Lane-ops/s of the VALU issue-rate probe (hammingdist_amd/hd_valu_probe:
+    v_bitop3 + popcount chains, no memory), or None when it is not built."""
+    exe = os.path.join(ROOT, "hammingdist_amd", "hd_valu_probe")
+    if not os.path.exists(exe):
+        return None
+    try:
+        out = subprocess.run([exe], capture_output=True, text=True, timeout=120, check=True)
+        return json.loads(out.stdout.strip().splitlines()[-1])
+    except (subprocess.SubprocessError, ValueError, IndexError) as e:
+        log(f"valu probe failed: {e}")
+        return None
+
+
 def run_large(args, torch, capi, workloads, np):
     spec = workloads.large(n=args.n, L=args.L)
     t = time.perf_counter()
@@ -256,6 +318,9 @@ def run_large(args, torch, capi, workloads, np):
                              "note": "same triangle; a tile stops once all its pairs reach "
                                      "max_distance"}
         log(f"large: early-exit kernel {ee_ms:.2f} ms")
+
+    if not args.no_sparse:
+        res["sparse_emit"] = run_sparse_emit(torch, capi, np, d_out, host, n, md, sh)
 
     if not args.no_refdist:
         # one query against every row of the resident packed rows (HBM scan)
@@ -503,6 +568,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-early-exit-run", action="store_true")
     ap.add_argument("--no-refdist", action="store_true")
+    ap.add_argument("--no-sparse", action="store_true")
     ap.add_argument("--no-wide", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -514,6 +580,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.headline_only:
         args.no_cpu_baseline = args.no_early_exit_run = args.no_refdist = True
+        args.no_sparse = True
         args.no_wide = args.no_batched = args.no_serving = args.no_e2e = True
 
     if args.gpus != 1:
@@ -537,7 +604,16 @@ def main() -> int:
 
     from hammingdist_amd import capi, workloads
 
+    ceiling = valu_ceiling()
+    if ceiling:
+        log(f"valu probe: {ceiling['lane_ops_per_s'] / 1e12:.2f} T lane-ops/s at "
+            f"{ceiling['shader_clock_ghz']:.3f} GHz")
     large, packed = run_large(args, torch, capi, workloads, np)
+    if ceiling:
+        rl = large["roofline"]
+        rl["measured_ceiling"] = ceiling["lane_ops_per_s"] / 1e12
+        rl["frac_of_measured_ceiling"] = rl["achieved"] / rl["measured_ceiling"]
+        rl["ceiling_shader_clock_ghz"] = ceiling["shader_clock_ghz"]
     result = {
         "metric": METRIC,
         "value": large["value"],
@@ -555,7 +631,7 @@ def main() -> int:
         "roofline": large["roofline"],
         "wall_s_timed": large["wall_s_timed"],
     }
-    for key in ("early_exit", "refdist_dense"):
+    for key in ("early_exit", "sparse_emit", "refdist_dense"):
         if key in large:
             result[key] = large[key]
     if not args.no_wide:

@@ -80,6 +80,16 @@ def build_engine() -> str:
     return out
 
 
+def build_probe() -> str:
+    """hd_valu_probe: the VALU issue-rate probe bench.py runs for the
+    roofline ceiling of the all-pairs kernels."""
+    out = os.path.join(HERE, "hd_valu_probe")
+    tmp = out + ".tmp"
+    _run([_hipcc(), *HIPCC_FLAGS, os.path.join(CSRC, "valu_probe.hip"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def _module_name() -> str:
     suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
     return "_hammingdist" + suffix
@@ -111,6 +121,7 @@ def build_host() -> str:
 def build() -> None:
     build_engine()
     build_host()
+    build_probe()
 
 
 if __name__ == "__main__":

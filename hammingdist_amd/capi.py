@@ -53,6 +53,8 @@ SIGNATURES = {
                                          C.c_int, vp]),
     "hd_consensus_diff": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64, u64p, u32p]),
     "hd_sparse_emit": (C.c_int, [vp, C.c_uint64, C.c_int, C.c_int64, vp, C.c_uint64, u64p]),
+    "hd_sparse_emit_dev": (C.c_int, [vp, C.c_uint64, C.c_int, C.c_int64, vp, C.c_uint64, u64p,
+                                     vp]),
     "hd_planes_layout": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, u64p,
                                    u32p, u32p, u64p]),
     "hd_nibble_mask_dev": (C.c_int, [vp, C.c_uint64, C.c_uint64, u32p, vp]),
@@ -183,6 +185,19 @@ def sparse_emit(cells, n: int, out_bytes: int, threshold: int, triplets, capacit
     st = lib().hd_sparse_emit(ptr(cells), n, out_bytes, threshold,
                               None if triplets is None else ptr(triplets), capacity,
                               C.byref(count))
+    if st not in (HD_OK, HD_ERR_CAPACITY) or (st == HD_ERR_CAPACITY and capacity):
+        check(st)
+    return count.value
+
+
+def sparse_emit_dev(d_cells, n: int, out_bytes: int, threshold: int, d_triplets,
+                    capacity: int, stream: int = 0) -> int:
+    """Device triangle -> device (i, j, d) triplets; returns the total found
+    (raises when it exceeds a non-zero capacity)."""
+    count = C.c_uint64(0)
+    st = lib().hd_sparse_emit_dev(ptr(d_cells), n, out_bytes, threshold,
+                                  None if d_triplets is None else ptr(d_triplets), capacity,
+                                  C.byref(count), stream)
     if st not in (HD_OK, HD_ERR_CAPACITY) or (st == HD_ERR_CAPACITY and capacity):
         check(st)
     return count.value

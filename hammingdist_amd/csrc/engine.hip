@@ -485,28 +485,120 @@ static void launch_refdist(const uint8_t *d_query, const uint8_t *d_db,
 // ---------------------------------------------------------------------------
 // sparse emit: cells <= threshold of a device triangle, row-major order
 // ---------------------------------------------------------------------------
-// Pass 1 counts qualifying cells per block of kEmitCells cells; an exclusive
-// scan of the counts gives each block its output offset; pass 2 writes the
-// triplets of a block in cell order using wavefront ballots.
-static constexpr uint32_t kEmitCells = 4096; // per 256-thread block
+// Each thread owns 16 consecutive cells (one to four 16-byte loads), each
+// 256-thread block kEmitCells = 4096 cells.  Pass 1 counts the qualifying
+// cells per block; a two-level scan on the device turns the counts into
+// block offsets (per 4096-block chunk, then across chunks); pass 2 re-reads
+// the cells, scans the per-thread hit counts inside the block and writes
+// each thread's hits in cell order.  Only the total goes to the host.
+static constexpr uint32_t kEmitPerThread = 16;
+static constexpr uint32_t kEmitCells = 256 * kEmitPerThread;
+static constexpr uint32_t kScanChunk = 256 * 16; // block counts per scan block
+
+// The 16 cells x0 .. x0+15 as uint32 values; returns how many of them
+// exist (cells past n_cells read as 0 and must be ignored).  vec: x0 is
+// 16-cell aligned in a 16-byte aligned buffer, so whole 16-byte loads where
+// all 16 cells exist.
+template <typename T>
+__device__ __forceinline__ uint32_t load16(const T *__restrict__ cells, uint64_t n_cells,
+                                           uint64_t x0, bool vec, uint32_t (&v)[16]) {
+  constexpr uint32_t kMask = sizeof(T) >= 4 ? 0xFFFFFFFFu : (1u << (8 * sizeof(T))) - 1u;
+  if (vec && x0 + 16 <= n_cells) {
+    constexpr int kLoads = 16 * sizeof(T) / 16;
+    const uint4 *p = reinterpret_cast<const uint4 *>(cells + x0);
+#pragma unroll
+    for (int l = 0; l < kLoads; ++l) {
+      const uint4 q = p[l];
+      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        constexpr int per = 4 / sizeof(T); // cells per 32-bit word
+#pragma unroll
+        for (int e = 0; e < per; ++e)
+          v[(l * 4 + k) * per + e] = (w[k] >> (8 * sizeof(T) * e)) & kMask;
+      }
+    }
+    return 16;
+  }
+#pragma unroll
+  for (int e = 0; e < 16; ++e) v[e] = x0 + e < n_cells ? (uint32_t)cells[x0 + e] : 0u;
+  return x0 >= n_cells ? 0u : (uint32_t)min<uint64_t>(16, n_cells - x0);
+}
+
+// Exclusive prefix of v over the 256 threads of the block; *total = the sum.
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *total) {
+  __shared__ uint64_t s_wave[4];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t up = __shfl_up(incl, off);
+    if (lane >= (uint32_t)off) incl += up;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint64_t before = 0;
+  for (uint32_t w = 0; w < wave; ++w) before += s_wave[w];
+  *total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+  __syncthreads();
+  return before + incl - v;
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_emit_count(const T *__restrict__ cells,
-                                                    uint64_t n_cells,
-                                                    uint32_t thr,
+                                                    uint64_t n_cells, uint32_t thr, int vec,
                                                     uint32_t *__restrict__ counts) {
-  const uint64_t base = (uint64_t)blockIdx.x * kEmitCells;
+  const uint64_t x0 = (uint64_t)blockIdx.x * kEmitCells + threadIdx.x * kEmitPerThread;
+  uint32_t v[16];
+  const uint32_t valid = load16(cells, n_cells, x0, vec != 0, v);
   uint32_t c = 0;
-  for (uint32_t o = threadIdx.x; o < kEmitCells; o += 256) {
-    const uint64_t x = base + o;
-    if (x < n_cells && (uint32_t)cells[x] <= thr) ++c;
-  }
+#pragma unroll
+  for (int e = 0; e < 16; ++e) c += ((uint32_t)e < valid && v[e] <= thr) ? 1u : 0u;
   __shared__ uint32_t s_sum[4];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
   if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = c;
   __syncthreads();
   if (threadIdx.x == 0) counts[blockIdx.x] = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
+}
+
+// Level 1: exclusive prefix of the block counts inside chunks of kScanChunk
+// (written over the counts) and each chunk's sum.
+__global__ __launch_bounds__(256) void k_emit_scan_local(uint32_t *__restrict__ counts,
+                                                         uint64_t blocks,
+                                                         uint64_t *__restrict__ chunk_sums) {
+  const uint64_t b0 = (uint64_t)blockIdx.x * kScanChunk + threadIdx.x * 16;
+  uint32_t c[16];
+  uint64_t sum = 0;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    c[e] = b0 + e < blocks ? counts[b0 + e] : 0u;
+    sum += c[e];
+  }
+  uint64_t total = 0;
+  uint64_t run = block_exclusive_scan(sum, &total);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    if (b0 + e < blocks) counts[b0 + e] = (uint32_t)run;
+    run += c[e];
+  }
+  if (threadIdx.x == 0) chunk_sums[blockIdx.x] = total;
+}
+
+// Level 2 (one block): exclusive prefix of the chunk sums, in place, and
+// the grand total in chunk_sums[chunks].
+__global__ __launch_bounds__(256) void k_emit_scan_chunks(uint64_t *__restrict__ chunk_sums,
+                                                          uint64_t chunks) {
+  uint64_t carry = 0;
+  for (uint64_t c0 = 0; c0 < chunks; c0 += 256) {
+    const uint64_t c = c0 + threadIdx.x;
+    const uint64_t v = c < chunks ? chunk_sums[c] : 0;
+    uint64_t total = 0;
+    const uint64_t ex = block_exclusive_scan(v, &total);
+    if (c < chunks) chunk_sums[c] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) chunk_sums[chunks] = carry;
 }
 
 __device__ __forceinline__ uint64_t row_of_cell(uint64_t x) {
@@ -519,37 +611,50 @@ __device__ __forceinline__ uint64_t row_of_cell(uint64_t x) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_emit_write(const T *__restrict__ cells,
-                                                    uint64_t n_cells, uint32_t thr,
-                                                    const uint64_t *__restrict__ offsets,
+                                                    uint64_t n_cells, uint32_t thr, int vec,
+                                                    const uint32_t *__restrict__ local_prefix,
+                                                    const uint64_t *__restrict__ chunk_prefix,
                                                     uint64_t capacity,
                                                     uint32_t *__restrict__ trip) {
-  const uint64_t base = (uint64_t)blockIdx.x * kEmitCells;
-  __shared__ uint32_t s_wave[4];
-  __shared__ uint64_t s_run;
-  if (threadIdx.x == 0) s_run = offsets[blockIdx.x];
-  __syncthreads();
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint32_t o = 0; o < kEmitCells; o += 256) {
-    const uint64_t x = base + o + threadIdx.x;
-    const bool hit = x < n_cells && (uint32_t)cells[x] <= thr;
-    const uint64_t bal = __ballot(hit);
-    const uint32_t before = __popcll(bal & ((1ull << lane) - 1));
-    if (lane == 0) s_wave[wave] = __popcll(bal);
-    __syncthreads();
-    uint64_t pos = s_run;
-    for (uint32_t w = 0; w < wave; ++w) pos += s_wave[w];
-    pos += before;
-    if (hit && pos < capacity) {
-      const uint64_t i = row_of_cell(x);
-      const uint64_t j = x - i * (i - 1) / 2;
-      trip[3 * pos + 0] = (uint32_t)i;
-      trip[3 * pos + 1] = (uint32_t)j;
-      trip[3 * pos + 2] = (uint32_t)cells[x];
+  const uint64_t x0 = (uint64_t)blockIdx.x * kEmitCells + threadIdx.x * kEmitPerThread;
+  uint32_t v[16];
+  const uint32_t valid = load16(cells, n_cells, x0, vec != 0, v);
+  uint32_t hits = 0, mask = 0;
+#pragma unroll
+  for (int e = 0; e < 16; ++e)
+    if ((uint32_t)e < valid && v[e] <= thr) {
+      ++hits;
+      mask |= 1u << e;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) s_run += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
-    __syncthreads();
+  // The block's triplets are contiguous in the output: they are assembled in
+  // LDS in cell order, then stored with consecutive lanes on consecutive
+  // words.
+  __shared__ uint32_t s_trip[3 * kEmitCells];
+  uint64_t total = 0;
+  uint32_t at = (uint32_t)block_exclusive_scan(hits, &total);
+  const uint64_t base = local_prefix[blockIdx.x] + chunk_prefix[blockIdx.x / kScanChunk];
+  if (mask) {
+    uint64_t i = row_of_cell(x0);
+    uint64_t row_start = i * (i - 1) / 2;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const uint64_t x = x0 + e;
+      while (x >= row_start + i) { // next row (rows of length i)
+        row_start += i;
+        ++i;
+      }
+      if ((mask >> e) & 1u) {
+        s_trip[3 * at + 0] = (uint32_t)i;
+        s_trip[3 * at + 1] = (uint32_t)(x - row_start);
+        s_trip[3 * at + 2] = v[e];
+        ++at;
+      }
+    }
   }
+  __syncthreads();
+  const uint64_t words = 3 * total, first = 3 * base, limit = 3 * capacity;
+  for (uint64_t w = threadIdx.x; w < words && first + w < limit; w += 256)
+    trip[first + w] = s_trip[w];
 }
 
 // Emits into device triplets; returns the total number of qualifying cells.
@@ -559,26 +664,26 @@ static uint64_t emit_t(const T *d_cells, uint64_t n_cells, uint32_t thr,
   const uint64_t blocks = (n_cells + kEmitCells - 1) / kEmitCells;
   if (!blocks) return 0;
   HD_REQUIRE(blocks < (1ull << 31), "triangle too large for the emit grid");
-  DeviceBuffer d_counts(blocks * sizeof(uint32_t));
-  hipLaunchKernelGGL((k_emit_count<T>), dim3((uint32_t)blocks), dim3(256), 0, s,
-                     d_cells, n_cells, thr, d_counts.get<uint32_t>());
+  const int vec = (reinterpret_cast<uintptr_t>(d_cells) % 16) == 0 ? 1 : 0;
+  const uint64_t chunks = (blocks + kScanChunk - 1) / kScanChunk;
+  DeviceBuffer d_counts(blocks * sizeof(uint32_t)), d_chunks((chunks + 1) * sizeof(uint64_t));
+  hipLaunchKernelGGL((k_emit_count<T>), dim3((uint32_t)blocks), dim3(256), 0, s, d_cells,
+                     n_cells, thr, vec, d_counts.get<uint32_t>());
   HD_HIP(hipGetLastError());
-  std::vector<uint32_t> counts(blocks);
-  HD_HIP(hipMemcpyAsync(counts.data(), d_counts.get(), blocks * 4,
+  hipLaunchKernelGGL(k_emit_scan_local, dim3((uint32_t)chunks), dim3(256), 0, s,
+                     d_counts.get<uint32_t>(), blocks, d_chunks.get<uint64_t>());
+  HD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_emit_scan_chunks, dim3(1), dim3(256), 0, s, d_chunks.get<uint64_t>(),
+                     chunks);
+  HD_HIP(hipGetLastError());
+  uint64_t total = 0;
+  HD_HIP(hipMemcpyAsync(&total, d_chunks.get<uint64_t>() + chunks, sizeof total,
                         hipMemcpyDeviceToHost, s));
   HD_HIP(hipStreamSynchronize(s));
-  std::vector<uint64_t> offsets(blocks);
-  uint64_t total = 0;
-  for (uint64_t b = 0; b < blocks; ++b) {
-    offsets[b] = total;
-    total += counts[b];
-  }
   if (total && capacity) {
-    DeviceBuffer d_off(blocks * sizeof(uint64_t));
-    HD_HIP(hipMemcpyAsync(d_off.get(), offsets.data(), blocks * 8,
-                          hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL((k_emit_write<T>), dim3((uint32_t)blocks), dim3(256), 0, s,
-                       d_cells, n_cells, thr, d_off.get<uint64_t>(), capacity, d_trip);
+    hipLaunchKernelGGL((k_emit_write<T>), dim3((uint32_t)blocks), dim3(256), 0, s, d_cells,
+                       n_cells, thr, vec, d_counts.get<uint32_t>(), d_chunks.get<uint64_t>(),
+                       capacity, d_trip);
     HD_HIP(hipGetLastError());
     HD_HIP(hipStreamSynchronize(s));
   }
@@ -920,6 +1025,16 @@ int hd_consensus_diff(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
   });
 }
 
+// Sparse emit of a device triangle into device triplets; returns the total.
+static uint64_t emit_dev(const void *d_cells, uint64_t nc, int out_bytes, uint32_t thr,
+                         uint32_t *d_trip, uint64_t capacity, hipStream_t s) {
+  if (out_bytes == 1)
+    return emit_t(static_cast<const uint8_t *>(d_cells), nc, thr, d_trip, capacity, s);
+  if (out_bytes == 2)
+    return emit_t(static_cast<const uint16_t *>(d_cells), nc, thr, d_trip, capacity, s);
+  return emit_t(static_cast<const uint32_t *>(d_cells), nc, thr, d_trip, capacity, s);
+}
+
 int hd_sparse_emit(const void *cells, uint64_t n, int out_bytes, int64_t threshold,
                    uint32_t *triplets, uint64_t capacity, uint64_t *count) {
   return guarded([&] {
@@ -937,18 +1052,35 @@ int hd_sparse_emit(const void *cells, uint64_t n, int out_bytes, int64_t thresho
     DeviceBuffer d_cells(nc * out_bytes);
     HD_HIP(hipMemcpyAsync(d_cells.get(), cells, nc * out_bytes, hipMemcpyHostToDevice, s));
     DeviceBuffer d_trip(std::max<uint64_t>(capacity, 1) * 12);
-    uint64_t total = 0;
-    if (out_bytes == 1)
-      total = emit_t(d_cells.get<uint8_t>(), nc, thr, d_trip.get<uint32_t>(), capacity, s);
-    else if (out_bytes == 2)
-      total = emit_t(d_cells.get<uint16_t>(), nc, thr, d_trip.get<uint32_t>(), capacity, s);
-    else
-      total = emit_t(d_cells.get<uint32_t>(), nc, thr, d_trip.get<uint32_t>(), capacity, s);
+    const uint64_t total =
+        emit_dev(d_cells.get(), nc, out_bytes, thr, d_trip.get<uint32_t>(), capacity, s);
     const uint64_t got = std::min(total, capacity);
     if (got) {
       HD_HIP(hipMemcpyAsync(triplets, d_trip.get(), got * 12, hipMemcpyDeviceToHost, s));
       HD_HIP(hipStreamSynchronize(s));
     }
+    *count = total;
+    if (total > capacity)
+      throw Error(HD_ERR_CAPACITY, "more cells below the threshold than capacity");
+  });
+}
+
+int hd_sparse_emit_dev(const void *d_cells, uint64_t n, int out_bytes, int64_t threshold,
+                       uint32_t *d_triplets, uint64_t capacity, uint64_t *count,
+                       void *stream) {
+  return guarded([&] {
+    HD_REQUIRE(count, "null count");
+    HD_REQUIRE(out_bytes == 1 || out_bytes == 2 || out_bytes == 4,
+               "out_bytes must be 1, 2 or 4");
+    *count = 0;
+    const uint64_t nc = tri_cells(n);
+    if (nc == 0 || threshold < 0) return;
+    HD_REQUIRE(d_cells, "null buffer");
+    HD_REQUIRE(d_triplets || capacity == 0, "null triplets");
+    ensure_device();
+    const uint32_t thr = threshold > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)threshold;
+    const uint64_t total = emit_dev(d_cells, nc, out_bytes, thr, d_triplets, capacity,
+                                    (hipStream_t)stream);
     *count = total;
     if (total > capacity)
       throw Error(HD_ERR_CAPACITY, "more cells below the threshold than capacity");

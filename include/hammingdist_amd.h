@@ -135,6 +135,15 @@ int hd_sparse_emit(const void *cells, uint64_t n, int out_bytes,
                    int64_t threshold, uint32_t *triplets, uint64_t capacity,
                    uint64_t *count);
 
+/* hd_sparse_emit on a device triangle into device triplets (d_triplets
+ * holds `capacity` triplets), ordered on `stream`; returns after the last
+ * write (the total is known on the host).  Same order, count and capacity
+ * rule as hd_sparse_emit; the device form of the dump_sparse selection
+ * (include/hamming/hamming.hh:79-114) for triangles that stay in HBM.      */
+int hd_sparse_emit_dev(const void *d_cells, uint64_t n, int out_bytes,
+                       int64_t threshold, uint32_t *d_triplets, uint64_t capacity,
+                       uint64_t *count, void *stream);
+
 /* ---- device-resident API (device pointers, hipStream_t passed as void*) -- */
 /* Layout of the bit-sliced planes buffer an encoded dataset needs:
  * planes[k][p][n_pad] uint32, k < n_words (32 sites each), p < n_planes.    */

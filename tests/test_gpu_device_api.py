@@ -82,3 +82,41 @@ def test_reference_distances_dev_matches_host(rng):
     torch.cuda.synchronize()
     want = [O.distance_char(seqs[0], s) for s in seqs[1:]]
     assert d_out.cpu().numpy().astype(np.uint32).tolist() == want
+
+
+@pytest.mark.parametrize("out_bytes", [1, 2, 4])
+def test_sparse_emit_dev_on_resident_triangle(out_bytes, rng):
+    # all-pairs into HBM, then the (i, j, d) emit of cells <= threshold from
+    # the resident triangle, row-major (include/hamming/hamming.hh:79-114)
+    n, L = 1300, 400
+    seqs = random_strings(rng, n, L, "ACGT-N")
+    base = seqs[0]
+    seqs = [base if k % 5 == 0 else s for k, s in enumerate(seqs)]  # some exact duplicates
+    packed = O.pack(O.as_chars(seqs))
+    md = {1: 255, 2: 65535, 4: 2**32 - 1}[out_bytes]
+    want_cells = O.all_pairs_dense(packed, out_bytes, md)
+    n_, bps = packed.shape
+    stream = torch.cuda.current_stream().cuda_stream
+    d_packed = torch.from_numpy(np.ascontiguousarray(packed)).cuda()
+    mask = capi.nibble_mask_dev(d_packed, n, bps, stream)
+    _, _, _, total = capi.planes_layout(n, bps, capi.HD_SEM_DENSE, mask)
+    d_planes = torch.empty(total, dtype=torch.int32, device="cuda")
+    capi.encode_dev(d_packed, n, bps, capi.HD_SEM_DENSE, mask, d_planes, stream)
+    d_cells = torch.zeros(want_cells.size * out_bytes, dtype=torch.uint8, device="cuda")
+    capi.all_pairs_dev(d_planes, n, bps, capi.HD_SEM_DENSE, mask, out_bytes, md, True, -1,
+                       d_cells, stream)
+    I, J = O.tri_index(n)
+    for thr in [0, 3, 150]:
+        sel = want_cells <= thr
+        count = capi.sparse_emit_dev(d_cells, n, out_bytes, thr, None, 0, stream)
+        assert count == int(sel.sum())
+        d_trip = torch.zeros(max(count, 1) * 3, dtype=torch.int32, device="cuda")
+        assert capi.sparse_emit_dev(d_cells, n, out_bytes, thr, d_trip, count, stream) == count
+        torch.cuda.synchronize()
+        trip = d_trip[:count * 3].cpu().numpy().view(np.uint32).reshape(-1, 3)
+        np.testing.assert_array_equal(trip[:, 0], I[sel])
+        np.testing.assert_array_equal(trip[:, 1], J[sel])
+        np.testing.assert_array_equal(trip[:, 2], want_cells[sel])
+        if count > 1:
+            with pytest.raises(capi.EngineError, match="capacity"):
+                capi.sparse_emit_dev(d_cells, n, out_bytes, thr, d_trip, count - 1, stream)

@@ -1,0 +1,113 @@
+"""The pair functions of the all-pairs kernel, checked exhaustively on the CPU.
+
+The tile kernel (hammingdist_amd/csrc/kernels.h) evaluates each semantics on
+bit planes with a fixed chain of v_bitop3 instructions.  Here the chain is
+evaluated bit by bit with the same truth tables for every pair of nibble
+classes and compared with the reference's per-site rules
+(src/hamming_impl.cc:131-140 for the dense rule; src/hamming_impl.cc:93-129
+for the sparse rule: codes differ and neither is '-').  The GPU parity tests
+check the kernels themselves; this file checks that the algebra they
+implement is the reference's rule on every input class.
+"""
+import itertools
+
+import pytest
+
+NIBBLE = {"inv": 0, "A": 1, "C": 2, "G": 4, "T": 8, "X": 3, "-": 15}
+
+
+def bitop3(x: int, y: int, z: int, imm: int) -> int:
+    """v_bitop3_b32 on single bits: the result is bit (4x + 2y + z) of imm
+    (truth tables written over src0 = 0xF0, src1 = 0xCC, src2 = 0xAA)."""
+    return (imm >> (4 * x + 2 * y + z)) & 1
+
+
+def dense_rule(a: str, b: str) -> int:
+    return int((NIBBLE[a] & NIBBLE[b]) == 0)
+
+
+def sparse_rule(a: str, b: str) -> int:
+    return int(a != b and a != "-" and b != "-")
+
+
+def dense3_planes(nib: int):
+    """Planes (g, c1, c2, c3) from the nibble bits, as the encode kernel
+    derives them."""
+    n0, n1, n2, n3 = (nib >> 0) & 1, (nib >> 1) & 1, (nib >> 2) & 1, (nib >> 3) & 1
+    g = n0 & n1
+    c1 = n0 | n2
+    c2 = n0 ^ n1
+    c3 = c2 | ((n2 | n3) & (1 - n0))
+    return g, c1, c2, c3
+
+
+DENSE3_TABLE = {"A": (0, 1, 1, 1), "C": (0, 0, 1, 1), "G": (0, 1, 0, 1), "T": (0, 0, 0, 1),
+                "-": (1, 1, 0, 0), "inv": (0, 0, 0, 0)}
+
+
+def test_dense3_encode_matches_table():
+    for cls, planes in DENSE3_TABLE.items():
+        assert dense3_planes(NIBBLE[cls]) == planes, cls
+
+
+def test_dense3_chain_with_corrections_is_the_dense_rule():
+    classes = list(DENSE3_TABLE)
+    for a, b in itertools.product(classes, classes):
+        g_i, c1_i, c2_i, c3_i = DENSE3_TABLE[a]
+        _, c1_j, c2_j, c3_j = DENSE3_TABLE[b]
+        s = bitop3(c1_i, c1_j, g_i, 0xBE)
+        s = bitop3(c2_i, c2_j, s, 0x41)
+        s = bitop3(c3_i, c3_j, s, 0x71)
+        site = s + (a == "inv") - (b == "-")
+        assert site == dense_rule(a, b), (a, b)
+
+
+def test_dense3_partial_sums_never_exceed_the_distance_bound():
+    # Per site the term is f3 + [row invalid] - [column '-'] = the dense
+    # rule (0 or 1), so every prefix of sites sums to the exact partial
+    # distance; the early-exit vote relies on this.
+    classes = list(DENSE3_TABLE)
+    for a, b in itertools.product(classes, classes):
+        g_i, c1_i, c2_i, c3_i = DENSE3_TABLE[a]
+        _, c1_j, c2_j, c3_j = DENSE3_TABLE[b]
+        s = bitop3(c3_i, c3_j, bitop3(c2_i, c2_j, bitop3(c1_i, c1_j, g_i, 0xBE), 0x41), 0x71)
+        assert 0 <= s + (a == "inv") - (b == "-") <= 1
+
+
+def class_code_table(present, bits):
+    """Class code of each class present (codes 0, 1, ... in the engine's
+    order: invalid, A, C, G, T, X) and '-' as the plane after the code bits."""
+    order = [c for c in ("inv", "A", "C", "G", "T", "X") if c in present]
+    table = {c: [(k >> b) & 1 for b in range(bits)] + [0] for k, c in enumerate(order)}
+    table["-"] = [0] * bits + [1]
+    return table
+
+
+@pytest.mark.parametrize("present", [("A", "C", "G", "T"),
+                                     ("inv", "A", "C", "G", "T"),
+                                     ("A", "C", "G", "T", "X"),
+                                     ("inv", "A", "C", "G", "T", "X"),
+                                     ("inv", "X")])
+def test_class_code_chain_is_the_sparse_rule(present):
+    bits = 2 if len(present) <= 4 else 3
+    table = class_code_table(present, bits)
+    classes = list(present) + ["-"]
+    for a, b in itertools.product(classes, classes):
+        pa, pb = table[a], table[b]
+        s = pa[0] ^ pb[0]
+        for p in range(1, bits):
+            s = bitop3(pa[p], pb[p], s, 0xBE)
+        s = bitop3(pa[bits], pb[bits], s, 0x02)
+        assert s == sparse_rule(a, b), (present, a, b)
+
+
+def test_one_hot_chain_is_the_dense_rule_on_nibbles():
+    # The 4-plane nibble code: s = a0 & b0, (ap & bp) | s, ~((a3 & b3) | s).
+    for x, y in itertools.product(range(16), range(16)):
+        a = [(x >> p) & 1 for p in range(4)]
+        b = [(y >> p) & 1 for p in range(4)]
+        s = a[0] & b[0]
+        for p in (1, 2):
+            s = bitop3(a[p], b[p], s, 0xEA)
+        s = bitop3(a[3], b[3], s, 0x15)
+        assert s == int((x & y) == 0)
