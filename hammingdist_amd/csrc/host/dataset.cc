@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <functional>
 #include <iostream>
 #include <memory>
 #include <sstream>
@@ -37,8 +38,8 @@ std::uint64_t common_length(const std::vector<std::string> &data) {
 
 } // namespace
 
-Route choose_route(const std::vector<std::string> &data, bool include_x, bool use_gpu,
-                   std::vector<GeneBlock> &packed, std::uint64_t bps) {
+Route choose_route(const Packer &pack, std::uint64_t n, std::uint64_t L, bool include_x,
+                   bool use_gpu, PackedRows &packed, std::uint64_t bps) {
   // include/hamming/hamming_impl.hh:99-102
   if (include_x && use_gpu)
     throw std::runtime_error(
@@ -47,7 +48,7 @@ Route choose_route(const std::vector<std::string> &data, bool include_x, bool us
   if (use_gpu) {
     // The reference GPU route is always the dense nibble kernel
     // (hamming_impl.hh:147-159, src/distance_cuda.cu:57-65).
-    packed = pack_all(data, false, bps);
+    packed = pack(false, bps);
     return r;
   }
   if (include_x) {
@@ -55,13 +56,12 @@ Route choose_route(const std::vector<std::string> &data, bool include_x, bool us
     r.semantics = HD_SEM_SPARSE_X;
     r.pack_include_x = true;
     r.sparse = true;
-    packed = pack_all(data, true, bps);
+    packed = pack(true, bps);
     return r;
   }
   // hamming_impl.hh:108-118: sparse when fewer than 0.5% of all entries
   // differ from the per-site consensus letter.
-  packed = pack_all(data, false, bps);
-  const std::uint64_t n = data.size(), L = common_length(data);
+  packed = pack(false, bps);
   std::uint64_t n_diff = 0;
   std::uint32_t mask = 0;
   check(hd_consensus_diff(packed.data(), n, bps, L, &n_diff, &mask));
@@ -75,26 +75,35 @@ Route choose_route(const std::vector<std::string> &data, bool include_x, bool us
 }
 
 template <typename T>
-std::vector<T> distances(std::vector<std::string> &data, bool include_x,
-                         bool clear_input_data, bool use_gpu, int max_distance) {
-  const std::size_t n = data.size();
+std::vector<T> distances_packed(const Packer &pack, std::uint64_t n, std::uint64_t L,
+                                bool include_x, bool use_gpu, int max_distance,
+                                const std::function<void()> &after_pack) {
   std::vector<T> result(n < 2 ? 0 : n * (n - 1) / 2, 0);
   const T max_dist = safe_int_cast<T>(max_distance);
   Timer timer;
-  const std::uint64_t L = common_length(data);
   const std::uint64_t bps = bytes_per_seq(L);
-  std::vector<GeneBlock> packed;
-  const Route route = choose_route(data, include_x, use_gpu, packed, bps);
+  PackedRows packed;
+  const Route route = choose_route(pack, n, L, include_x, use_gpu, packed, bps);
   std::cout << "# hammingdist :: Using GPU with "
             << (route.sparse ? "sparse" : "dense") << " distance semantics..."
             << std::endl;
-  if (clear_input_data) data.clear();
+  if (after_pack) after_pack();
   timer.lap("pre-processing");
   if (n >= 2)
     check(hd_all_pairs(packed.data(), n, bps, route.semantics, sizeof(T), max_dist,
                        result.data()));
   timer.lap("distance calculation", true);
   return result;
+}
+
+template <typename T>
+std::vector<T> distances(std::vector<std::string> &data, bool include_x,
+                         bool clear_input_data, bool use_gpu, int max_distance) {
+  const Packer pack = [&](bool x, std::uint64_t bps) { return pack_all(data, x, bps); };
+  return distances_packed<T>(pack, data.size(), common_length(data), include_x, use_gpu,
+                             max_distance, [&] {
+                               if (clear_input_data) data.clear();
+                             });
 }
 
 template std::vector<std::uint8_t> distances<std::uint8_t>(std::vector<std::string> &, bool,
@@ -213,13 +222,53 @@ template <typename T> DataSet<T> from_lower_triangular(const std::string &filena
 template DataSet<std::uint8_t> from_lower_triangular<std::uint8_t>(const std::string &);
 template DataSet<std::uint16_t> from_lower_triangular<std::uint16_t>(const std::string &);
 
-// include/hamming/hamming.hh:168-178
+namespace {
+
+// Rows of a FASTA file as read_fasta gives them (with dedup when asked),
+// checked as validate_data checks them (src/hamming_impl.cc:80-91).
+struct FastaRows {
+  std::vector<std::size_t> rows;    // record of each row
+  std::vector<std::size_t> indices; // dedup: row of each record
+  std::uint64_t L = 0;
+};
+
+FastaRows fasta_rows(const FastaRecords &recs, bool remove_duplicates) {
+  FastaRows r;
+  r.rows.resize(recs.size());
+  for (std::size_t k = 0; k < r.rows.size(); ++k) r.rows[k] = k;
+  if (remove_duplicates) r.indices = recs.dedup(r.rows);
+  return r;
+}
+
+void validate_rows(const FastaRecords &recs, FastaRows &r) {
+  if (r.rows.empty() || recs.length(r.rows[0]) == 0)
+    throw std::runtime_error("Error: Empty sequence");
+  r.L = recs.length(r.rows[0]);
+  for (std::size_t k : r.rows)
+    if (recs.length(k) != r.L)
+      throw std::runtime_error("Error: Sequences do not all have the same length");
+}
+
+} // namespace
+
+// include/hamming/hamming.hh:168-178.  The records are packed straight from
+// the mapped file; no per-sequence strings are made.
 template <typename T>
 DataSet<T> from_fasta(const std::string &filename, bool include_x, bool remove_duplicates,
                       std::size_t n, bool use_gpu, int max_distance) {
-  auto fa = read_fasta(filename, remove_duplicates, n);
-  return DataSet<T>(fa.data, include_x, true, std::move(fa.sequence_indices), use_gpu,
-                    max_distance);
+  FastaRecords recs(filename, n);
+  FastaRows fr = fasta_rows(recs, remove_duplicates);
+  validate_rows(recs, fr);
+  const Packer pack = [&](bool x, std::uint64_t bps) {
+    PackedRows out(fr.rows.size() * bps);
+    recs.pack_rows(fr.rows, pack_table(x), out.data(), bps);
+    return out;
+  };
+  DataSet<T> ds(distances_packed<T>(pack, fr.rows.size(), fr.L, include_x, use_gpu,
+                                    max_distance, {}));
+  ds.nsamples = fr.rows.size();
+  ds.sequence_indices = std::move(fr.indices);
+  return ds;
 }
 template DataSet<std::uint8_t> from_fasta<std::uint8_t>(const std::string &, bool, bool,
                                                         std::size_t, bool, int);
@@ -267,14 +316,19 @@ void from_fasta_to_lower_triangular(const std::string &input_filename,
         "from_fasta_to_lower_triangular is currently only available on GPU");
   std::cout << "# hammingdist :: Using GPU..." << std::endl;
   Timer timer;
-  auto fa = read_fasta(input_filename, remove_duplicates, n);
-  const std::size_t count = fa.data.size();
-  if (count < 2) return; // no cells: the reference writes nothing
-  validate_data(fa.data);
-  const std::uint64_t bps = bytes_per_seq(fa.data[0].size());
-  std::vector<GeneBlock> packed = pack_all(fa.data, false, bps);
-  fa.data.clear();
-  fa.data.shrink_to_fit();
+  PackedRows packed;
+  std::size_t count = 0;
+  std::uint64_t bps = 0;
+  {
+    FastaRecords recs(input_filename, n);
+    FastaRows fr = fasta_rows(recs, remove_duplicates);
+    count = fr.rows.size();
+    if (count < 2) return; // no cells: the reference writes nothing
+    validate_rows(recs, fr);
+    bps = bytes_per_seq(fr.L);
+    packed.resize(count * bps);
+    recs.pack_rows(fr.rows, pack_table(false), packed.data(), bps);
+  }
   timer.lap("pre-processing");
   const std::uint16_t max_dist = max_distance > 65535 ? std::uint16_t{65535}
                                                       : static_cast<std::uint16_t>(max_distance);
@@ -311,25 +365,17 @@ ReferenceDistIntType distance(const std::string &seq0, const std::string &seq1,
 // count 1, while the reference does not visit those sites.
 std::vector<ReferenceDistIntType> fasta_reference_distances(
     const std::string &reference_sequence, const std::string &fasta_file, bool include_x) {
-  std::ifstream probe(fasta_file);
-  if (!probe) throw std::runtime_error("Error: Failed to open file '" + fasta_file + "'");
-  probe.close();
-  FastaData fa;
-  {
-    // All records, no limit, no dedup: the reference's loop at
-    // src/hamming.cc:115-139 reads the file the way read_fasta does.
-    fa = read_fasta(fasta_file, false, 0);
-  }
+  // All records, no limit, no dedup: the reference's loop at
+  // src/hamming.cc:115-139 reads the file the way read_fasta does.
+  FastaRecords recs(fasta_file, 0);
   const std::uint64_t Lref = reference_sequence.size();
-  const std::size_t m = fa.data.size();
+  const std::size_t m = recs.size();
   std::vector<ReferenceDistIntType> out(m, 0);
-  if (m == 0 || Lref == 0) {
-    for (std::size_t k = 0; k < m; ++k)
-      if (!fa.data[k].empty())
-        throw std::runtime_error("Error: sequence " + std::to_string(k) +
-                                 " is longer than the reference sequence");
-    return out;
-  }
+  for (std::size_t k = 0; k < m; ++k)
+    if (recs.length(k) > Lref)
+      throw std::runtime_error("Error: sequence " + std::to_string(k) +
+                               " is longer than the reference sequence");
+  if (m == 0 || Lref == 0) return out;
   const auto lut = lookup_table(include_x);
   const std::uint64_t bps = bytes_per_seq(Lref);
   std::vector<GeneBlock> query(bps);
@@ -339,18 +385,13 @@ std::vector<ReferenceDistIntType> fasta_reference_distances(
   for (std::uint64_t s = Lref; s-- > 0;)
     invalid_after[s] = invalid_after[s + 1] +
                        (lut[static_cast<unsigned char>(reference_sequence[s])] == 0 ? 1u : 0u);
-  for (std::size_t k = 0; k < m; ++k)
-    if (fa.data[k].size() > Lref)
-      throw std::runtime_error("Error: sequence " + std::to_string(k) +
-                               " is longer than the reference sequence");
-  std::vector<GeneBlock> db(m * bps);
-  parallel_for(m, [&](std::uint64_t b, std::uint64_t e) {
-    for (std::uint64_t k = b; k < e; ++k)
-      pack_row(fa.data[k].data(), fa.data[k].size(), lut, db.data() + k * bps, bps);
-  });
+  std::vector<std::size_t> all(m);
+  for (std::size_t k = 0; k < m; ++k) all[k] = k;
+  PackedRows db(m * bps);
+  recs.pack_rows(all, pack_table(include_x), db.data(), bps);
   check(hd_reference_distances(query.data(), db.data(), m, bps, nullptr, m,
                                include_x ? HD_SEM_CHAR_X : HD_SEM_DENSE, out.data()));
-  for (std::size_t k = 0; k < m; ++k) out[k] -= invalid_after[fa.data[k].size()];
+  for (std::size_t k = 0; k < m; ++k) out[k] -= invalid_after[recs.length(k)];
   return out;
 }
 

@@ -7,7 +7,9 @@
 
 #include <array>
 #include <cstdint>
+#include <functional>
 #include <limits>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -19,17 +21,40 @@ using GeneBlock = std::uint8_t;
 using ReferenceDistIntType = std::uint32_t;
 using DefaultDistIntType = std::uint8_t;
 
+// Allocator that leaves new elements uninitialised: packed rows are always
+// written in full before they are read, so a zero fill would be wasted.
+template <typename T> struct NoInitAllocator : std::allocator<T> {
+  template <typename U> struct rebind {
+    using other = NoInitAllocator<U>;
+  };
+  NoInitAllocator() = default;
+  template <typename U> NoInitAllocator(const NoInitAllocator<U> &) noexcept {}
+  template <typename U> void construct(U *) noexcept {}
+  template <typename U, typename... A> void construct(U *p, A &&...a) {
+    ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+  }
+};
+using PackedRows = std::vector<GeneBlock, NoInitAllocator<GeneBlock>>;
+
 // ---- encoding (src/hamming_impl.cc:23-44, 244-267) ------------------------
 std::array<GeneBlock, 256> lookup_table(bool include_x = false);
+// lookup_table for two characters at once: entry (c1 << 8) | c0 is the
+// packed byte of sites c0 (low nibble) and c1 (high nibble).
+struct PackTable {
+  std::array<GeneBlock, 256> lut;
+  std::vector<GeneBlock> pair;
+};
+const PackTable &pack_table(bool include_x);
 // Bytes of one packed row of L sites: two sites per byte, padded to 8 bytes.
 std::uint64_t bytes_per_seq(std::uint64_t L);
 // Packs one sequence (from_string) into bps bytes; bytes past the sequence
 // are 0xFF ('-' in both nibbles).
 void pack_row(const char *s, std::uint64_t L, const std::array<GeneBlock, 256> &lut,
               GeneBlock *out, std::uint64_t bps);
+void pack_row(const char *s, std::uint64_t L, const PackTable &t, GeneBlock *out,
+              std::uint64_t bps);
 // Packs every sequence into n * bps contiguous bytes (parallel over rows).
-std::vector<GeneBlock> pack_all(const std::vector<std::string> &data, bool include_x,
-                                std::uint64_t bps);
+PackedRows pack_all(const std::vector<std::string> &data, bool include_x, std::uint64_t bps);
 
 // ---- FASTA ingest (src/hamming_impl.cc:80-91, 200-242; src/hamming.cc:76-102)
 struct FastaData {
@@ -38,6 +63,46 @@ struct FastaData {
 };
 FastaData read_fasta(const std::string &filename, bool remove_duplicates = false,
                      std::size_t n = 0);
+
+// The records of a FASTA file (read_fasta's rules) without copying them:
+// the file is mapped, record ranges, lengths and dedup hashes are found in
+// parallel, and records are packed straight from the mapping.
+class FastaRecords {
+public:
+  struct Span {
+    std::uint64_t begin, end; // bytes of the record's lines in the file
+  };
+  FastaRecords(const std::string &filename, std::size_t n);
+  ~FastaRecords();
+  FastaRecords(const FastaRecords &) = delete;
+  FastaRecords &operator=(const FastaRecords &) = delete;
+
+  std::size_t size() const { return spans_.size(); }
+  // Sequence length of record k (its characters without line ends).
+  std::uint64_t length(std::size_t k) const { return lengths_[k]; }
+  std::string text(std::size_t k) const;
+  // Row k packed as from_string packs it, into bps bytes.
+  void pack(std::size_t k, const PackTable &t, GeneBlock *out, std::uint64_t bps,
+            std::string &scratch) const;
+  // records[r] packed into out + r * bps, in parallel.
+  void pack_rows(const std::vector<std::size_t> &records, const PackTable &t, GeneBlock *out,
+                 std::uint64_t bps) const;
+  // Per record, the row of its first identical record among the unique
+  // ones; `unique` receives the record index of every unique row.
+  std::vector<std::size_t> dedup(std::vector<std::size_t> &unique) const;
+
+private:
+  std::uint64_t hash(std::size_t k) const;
+  bool same_text(std::size_t a, std::size_t b) const;
+
+  int fd_ = -1;
+  void *map_ = nullptr;
+  std::string buffer_;
+  const char *data_ = nullptr;
+  std::uint64_t size_ = 0;
+  std::vector<Span> spans_;
+  std::vector<std::uint64_t> lengths_;
+};
 std::vector<std::size_t> fasta_sequence_indices(const std::string &fasta_file,
                                                 std::size_t n = 0);
 void validate_data(const std::vector<std::string> &data);
@@ -61,8 +126,17 @@ struct Route {
   bool pack_include_x = false;
   bool sparse = false;
 };
-Route choose_route(const std::vector<std::string> &data, bool include_x, bool use_gpu,
-                   std::vector<GeneBlock> &packed, std::uint64_t bps);
+// Packs the dataset's rows with the include_x table or not, bps bytes each.
+using Packer = std::function<PackedRows(bool include_x, std::uint64_t bps)>;
+Route choose_route(const Packer &pack, std::uint64_t n, std::uint64_t L, bool include_x,
+                   bool use_gpu, PackedRows &packed, std::uint64_t bps);
+
+// distances<T> on rows a Packer provides (n rows of L sites); after_pack
+// runs once the rows are packed.
+template <typename T>
+std::vector<T> distances_packed(const Packer &pack, std::uint64_t n, std::uint64_t L,
+                                bool include_x, bool use_gpu, int max_distance,
+                                const std::function<void()> &after_pack);
 
 // distances<T>, include/hamming/hamming_impl.hh:68-176
 template <typename T>

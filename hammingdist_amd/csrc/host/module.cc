@@ -144,6 +144,16 @@ PYBIND11_MODULE(_hammingdist, m) {
   // ---- engine extras (no counterpart in the reference) -------------------
   m.def("_bytes_per_seq", &bytes_per_seq, py::arg("L"));
   m.def(
+      "_read_fasta",
+      [](const std::string &filename, bool remove_duplicates, std::size_t n) {
+        FastaData fa = read_fasta(filename, remove_duplicates, n);
+        py::list data;
+        for (const auto &s : fa.data) data.append(py::bytes(s));
+        return py::make_tuple(data, fa.sequence_indices);
+      },
+      py::arg("filename"), py::arg("remove_duplicates") = false, py::arg("n") = 0,
+      "(sequences as bytes, sequence indices) as read_fasta parses the file");
+  m.def(
       "_pack",
       [](const std::vector<std::string> &data, bool include_x) {
         const std::uint64_t L = data.empty() ? 0 : data[0].size();

@@ -35,6 +35,70 @@ def test_sequence_indices_match_oracle(tmp_path, content, n):
     assert hd.fasta_sequence_indices(p, n).dtype == np.uint64
 
 
+FASTA_EDGE_CASES = [
+    ">a\nAC\n>b\nAC\n>c\nGT\n",
+    ">a\nAC\n>b\n",
+    ">a\nAC\n>b",
+    ">a\nAC\n\nGT\n>b\nACGT\n>c\nTT\n",
+    "",
+    "\n",
+    "no newline at all",
+    "no header\nAC\n>x\nAC",
+    ">a\r\nAC\r\n>b\r\nAC\r\n",
+    ">only\n",
+    "\n>a\nAC\n",
+    ">a\n>b\nAC\n>c\n\n",
+    ">a\nAC>b\nGT\n",
+    ">a\nAC\nGT",
+    ">a\n\n\n",
+]
+
+
+@pytest.mark.parametrize("content", FASTA_EDGE_CASES)
+@pytest.mark.parametrize("n", [0, 1, 2, 3])
+@pytest.mark.parametrize("dedup", [False, True])
+def test_read_fasta_records_match_oracle(tmp_path, content, n, dedup):
+    # src/hamming_impl.cc:200-242: records, their text and the dedup rows
+    from hammingdist_amd import _native
+
+    p = str(tmp_path / "f.fa")
+    open(p, "w", newline="").write(content)
+    data, idx = _native._read_fasta(p, dedup, n)
+    want_data, want_idx = read_fasta(p, dedup, n)
+    assert [d.decode("latin-1") for d in data] == want_data
+    assert list(idx) == want_idx
+
+
+def test_read_fasta_large_random_structure_matches_oracle(tmp_path, rng):
+    # Several MiB, so the record splitter works on many chunks: random line
+    # lengths, header lines, '>' inside lines, empty lines, CR, duplicates.
+    lines = []
+    pool = ["".join(rng.choice(list("ACGT-N"), size=int(rng.integers(0, 90))))
+            for _ in range(40)]
+    for _ in range(140_000):
+        r = rng.random()
+        if r < 0.08:
+            lines.append(">h" + str(int(rng.integers(0, 1000))))
+        elif r < 0.10:
+            lines.append("")
+        elif r < 0.12:
+            lines.append("AC>GT\r")
+        else:
+            lines.append(pool[int(rng.integers(0, len(pool)))])
+    content = "\n".join(lines) + ("\n" if rng.random() < 0.5 else "")
+    p = str(tmp_path / "big.fa")
+    open(p, "w", newline="").write(content)
+    assert len(content) > 4 << 20
+    from hammingdist_amd import _native
+
+    for dedup, n in [(False, 0), (True, 0), (True, 5000)]:
+        data, idx = _native._read_fasta(p, dedup, n)
+        want_data, want_idx = read_fasta(p, dedup, n)
+        assert len(data) == len(want_data)
+        assert [d.decode("latin-1") for d in data] == want_data
+        assert list(idx) == want_idx
+
+
 def test_sequence_indices_random_dedup(tmp_path, rng):
     base = random_strings(rng, 30, 40, "ACGT-N")
     seqs = [base[k] for k in rng.integers(0, 30, size=200)]
