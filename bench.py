@@ -112,7 +112,20 @@ def cpu_baseline(packed, L: int, target_s: float = 12.0):
                   f"OpenMP schedule(static,1), {used} threads",
         "early_exit_max_dist_2": {"value": cells2 * L / dt2, "pairs": cells2,
                                   "seconds": round(dt2, 3)},
+        "cpu_model": cpu_model(),
+        "host_cpus_visible": os.cpu_count(),
     }
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def measured_traffic(n: int, L: int, md: int):
@@ -157,6 +170,31 @@ class Timer:
         b.record(self.stream)
         t.cuda.synchronize()
         return a.elapsed_time(b) / reps
+
+
+def run_host_api(capi, np, packed, L, host_head):
+    """hd_all_pairs on host buffers (the reference's distances_cuda_8bit /
+    _16bit boundary): upload of the packed rows, mask, encode, all-pairs with
+    the early exit on, download of the whole triangle: the PCIe-inclusive
+    rate of the same workload."""
+    n, bps = packed.shape
+    cells = n * (n - 1) // 2
+    out = {}
+    for ob, md in ((1, 2), (2, 65535)):
+        dst = np.empty(cells, np.uint8 if ob == 1 else np.uint16)
+        capi.all_pairs(packed, n, bps, capi.HD_SEM_DENSE, ob, md, dst)  # warm-up
+        t = time.perf_counter()
+        capi.all_pairs(packed, n, bps, capi.HD_SEM_DENSE, ob, md, dst)
+        secs = time.perf_counter() - t
+        if ob == 1:
+            assert np.array_equal(dst[: host_head.size], host_head), "host API != device API"
+        else:
+            assert (np.minimum(dst[: host_head.size], 2) == host_head).all()
+        out[f"uint{8 * ob}_max_distance_{md}"] = {
+            "seconds": secs, "site_comparisons_per_s": cells * L / secs,
+            "note": "host packed rows in, host triangle out (PCIe both ways), early exit on"}
+        log(f"host API uint{8 * ob} max_distance {md}: {secs:.3f} s")
+    return out
 
 
 def run_sparse_emit(torch, capi, np, d_cells, host_head, n, md, sh):
@@ -321,6 +359,9 @@ def run_large(args, torch, capi, workloads, np):
 
     if not args.no_sparse:
         res["sparse_emit"] = run_sparse_emit(torch, capi, np, d_out, host, n, md, sh)
+
+    if not args.no_host_api:
+        res["host_api"] = run_host_api(capi, np, packed, args.L, host)
 
     if not args.no_refdist:
         # one query against every row of the resident packed rows (HBM scan)
@@ -569,6 +610,7 @@ def main() -> int:
     ap.add_argument("--no-early-exit-run", action="store_true")
     ap.add_argument("--no-refdist", action="store_true")
     ap.add_argument("--no-sparse", action="store_true")
+    ap.add_argument("--no-host-api", action="store_true")
     ap.add_argument("--no-wide", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -580,7 +622,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.headline_only:
         args.no_cpu_baseline = args.no_early_exit_run = args.no_refdist = True
-        args.no_sparse = True
+        args.no_sparse = args.no_host_api = True
         args.no_wide = args.no_batched = args.no_serving = args.no_e2e = True
 
     if args.gpus != 1:
@@ -631,7 +673,7 @@ def main() -> int:
         "roofline": large["roofline"],
         "wall_s_timed": large["wall_s_timed"],
     }
-    for key in ("early_exit", "sparse_emit", "refdist_dense"):
+    for key in ("early_exit", "sparse_emit", "host_api", "refdist_dense"):
         if key in large:
             result[key] = large[key]
     if not args.no_wide:
