@@ -572,6 +572,7 @@ def run_serving(args, torch, capi, workloads, np):
     warm = reqs[: max(8, len(reqs) // 10)]
     serving.closed_loop(srv, warm)
     _, secs, lat_c = serving.closed_loop(srv, reqs)
+    submit_s = serving.closed_loop.submit_seconds
     capacity = len(reqs) / secs
     # open loop at 70% of capacity
     rate = 0.7 * capacity
@@ -587,7 +588,10 @@ def run_serving(args, torch, capi, workloads, np):
            "offered_rate": rate, "achieved_requests_per_s": len(reqs) / secs_o,
            "open_loop": serving.percentiles(lat_o),
            "query_rows_GB_per_s_closed": rows_scanned * db.shape[1] / secs / 1e9,
+           "closed_loop_seconds": secs, "closed_loop_submit_seconds": submit_s,
            "stats": st}
+    log(f"serving: closed loop {secs * 1e3:.0f} ms ({submit_s * 1e3:.0f} ms submitting), "
+        f"{rows_scanned * db.shape[1] / secs / 1e12:.2f} TB/s of rows scanned")
     log(f"serving: capacity {capacity:.0f} req/s; open loop at {rate:.0f}/s "
         f"p50 {res['open_loop']['p50_ms']:.2f} ms p99 {res['open_loop']['p99_ms']:.2f} ms")
     return res

@@ -369,20 +369,20 @@ struct hd_server {
         Request &r = *reqs[pending.front()];
         const uint64_t d = dev_arena.alloc(r.lay.dev);
         if (d == UINT64_MAX) break;
-        const uint64_t h = host_arena.alloc(r.lay.host);
-        if (h == UINT64_MAX) {
-          dev_arena.release(d);
-          break;
+        if (r.h_off == UINT64_MAX) { // inputs not staged at submit
+          const uint64_t h = host_arena.alloc(r.lay.host);
+          if (h == UINT64_MAX) {
+            dev_arena.release(d);
+            break;
+          }
+          r.h_off = h;
+          uint8_t *hb = host_arena.at(h);
+          if (r.kind == QUERY) std::memcpy(hb, r.query.data(), bpr);
+          std::memcpy(hb + r.lay.h_rows, r.rows.data(), r.m * 8);
+          std::vector<uint64_t>().swap(r.rows);
+          std::vector<uint8_t>().swap(r.query);
         }
         r.d_off = d;
-        r.h_off = h;
-        uint8_t *hb = host_arena.at(h);
-        if (r.kind == QUERY) {
-          std::memcpy(hb, r.query.data(), bpr);
-          std::memcpy(hb + r.lay.h_rows, r.rows.data(), r.m * 8);
-        } else {
-          std::memcpy(hb + r.lay.h_rows, r.rows.data(), r.m * 8);
-        }
         r.state = HD_REQ_ADMITTED;
         admitted.push_back(r.id);
         pending.pop_front();
@@ -483,18 +483,54 @@ int hd_server_destroy(hd_server *s) {
   return guarded([&] { delete s; });
 }
 
-static int submit(hd_server *s, std::unique_ptr<hd_server::Request> r, uint64_t *id) {
+// The caller's thread stages the request's inputs (query, row list) in the
+// pinned host arena when there is room, outside the server lock, so the
+// worker thread only admits by device footprint and launches; when the host
+// arena is full the inputs are kept in the request and staged at admission.
+static int submit(hd_server *s, std::unique_ptr<hd_server::Request> r, const uint8_t *query,
+                  const uint64_t *rows, uint64_t *id) {
   return guarded([&] {
     HD_REQUIRE(s && id, "null server or id");
-    for (uint64_t row : r->rows) HD_REQUIRE(row < s->n_db, "row index out of range");
     if (r->kind == hd_server::ALLPAIRS) r->g = geometry(r->m, s->bpr, r->semantics, s->db_mask);
     r->lay = s->layout(*r);
     if (r->lay.dev > s->dev_arena.size() || r->lay.host > s->host_arena.size())
       throw Error(HD_ERR_CAPACITY, "request larger than the server's memory budget");
     r->t_submit = std::chrono::steady_clock::now();
+    uint8_t *staged = nullptr;
     {
       std::lock_guard<std::mutex> lk(s->mu);
       if (!s->fatal.empty()) throw Error(HD_ERR_HIP, s->fatal);
+      const uint64_t h = s->host_arena.alloc(r->lay.host);
+      if (h != UINT64_MAX) {
+        r->h_off = h;
+        staged = s->host_arena.at(h);
+      }
+    }
+    // One pass over the row list: range check and copy.
+    uint64_t *dst = nullptr;
+    if (staged) {
+      if (r->kind == hd_server::QUERY) std::memcpy(staged, query, s->bpr);
+      dst = reinterpret_cast<uint64_t *>(staged + r->lay.h_rows);
+    } else {
+      if (r->kind == hd_server::QUERY) r->query.assign(query, query + s->bpr);
+      r->rows.resize(r->m);
+      dst = r->rows.data();
+    }
+    uint64_t worst = 0;
+    for (uint64_t k = 0; k < r->m; ++k) {
+      const uint64_t v = rows[k];
+      worst = std::max(worst, v);
+      dst[k] = v;
+    }
+    if (worst >= s->n_db) {
+      if (staged) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->host_arena.release(r->h_off);
+      }
+      throw Error(HD_ERR_INVALID, "row index out of range");
+    }
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
       r->id = s->next_id++;
       r->state = HD_REQ_PENDING;
       *id = r->id;
@@ -516,9 +552,7 @@ int hd_server_submit_query(hd_server *s, const uint8_t *query, const uint64_t *r
   r->kind = hd_server::QUERY;
   r->semantics = semantics;
   r->m = m;
-  r->query.assign(query, query + s->bpr);
-  r->rows.assign(rows, rows + m);
-  return submit(s, std::move(r), id);
+  return submit(s, std::move(r), query, rows, id);
 }
 
 int hd_server_submit_all_pairs(hd_server *s, const uint64_t *rows, uint64_t n, int semantics,
@@ -534,8 +568,7 @@ int hd_server_submit_all_pairs(hd_server *s, const uint64_t *rows, uint64_t n, i
   r->m = n;
   r->out_bytes = out_bytes;
   r->max_dist = max_dist;
-  r->rows.assign(rows, rows + n);
-  return submit(s, std::move(r), id);
+  return submit(s, std::move(r), nullptr, rows, id);
 }
 
 int hd_server_wait(hd_server *s, uint64_t id, double timeout_s, int *state) {

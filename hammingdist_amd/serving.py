@@ -85,10 +85,16 @@ def run_alone(server: "capi.Server", reqs: list[Request]):
     return out
 
 
-def closed_loop(server: "capi.Server", reqs: list[Request]):
-    """All requests at once; returns (results, seconds, latencies)."""
+def closed_loop(server: "capi.Server", reqs: list[Request], submitters: int = 4):
+    """All requests at once, submitted by `submitters` client threads (each
+    submit stages its inputs in the caller's thread, so several clients
+    submit in parallel); returns (results, seconds, latencies)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     t = time.perf_counter()
-    ids = [submit(server, r) for r in reqs]
+    with ThreadPoolExecutor(max_workers=submitters) as ex:
+        ids = list(ex.map(lambda r: submit(server, r), reqs))
+    closed_loop.submit_seconds = time.perf_counter() - t
     res, lat = [], []
     for rid in ids:
         assert server.wait(rid) == capi.HD_REQ_DONE
