@@ -47,6 +47,25 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+# Oracle comparisons of samples of every leg's output.  The legs only record
+# the samples; the checks run in the CPU-baseline leg, the one place of this
+# benchmark that loads the oracle (it is the checker, never the thing timed).
+CHECKS: list = []
+
+
+def defer_check(name: str, fn) -> None:
+    CHECKS.append((name, fn))
+
+
+def run_checks() -> dict:
+    O = oracle()
+    done = []
+    for name, fn in CHECKS:
+        fn(O)
+        done.append(name)
+    return {"passed": done}
+
+
 def oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
@@ -306,15 +325,20 @@ def run_large(args, torch, capi, workloads, np):
     kern_ms = sum(kernel_ms) / len(kernel_ms)
     log(f"large: {args.steps} steps {ms_per_step:.2f} ms/step, all-pairs kernel {kern_ms:.2f} ms")
 
-    # spot check of cells against the oracle's dense pair kernel
-    O = oracle()
+    # sample of cells for the oracle's dense pair kernel
     rng = np.random.default_rng(0)
     host = d_out[: 1 << 20].cpu().numpy()
+    sample = []
     for x in rng.integers(0, min(cells, 1 << 20), size=64):
         i = tri_row(int(x))
         j = int(x) - i * (i - 1) // 2
-        want = O.distance_dense(packed[i], packed[j], md)
-        assert int(host[x]) == want, (i, j, int(host[x]), want)
+        sample.append((packed[i].copy(), packed[j].copy(), int(host[x])))
+
+    def check_large(O, sample=sample, md=md):
+        for a, b, got in sample:
+            assert got == O.distance_dense(a, b, md), "large: cell differs from the oracle"
+
+    defer_check("large all-pairs cells (64)", check_large)
 
     site_cmp = cells * args.L
     ops = cells * W * (OPS_PER_WORD_DENSE3 if dense3 else OPS_PER_WORD[P])
@@ -371,8 +395,13 @@ def run_large(args, torch, capi, workloads, np):
         ms = tm.time(lambda: capi.reference_distances_dev(q, d_packed, n, bps, None, n,
                                                           capi.HD_SEM_DENSE, d_rd, sh), 20)
         got = d_rd[:64].cpu().numpy().astype(np.uint32)
-        for r in range(64):
-            assert int(got[r]) == O.distance_dense(packed[1], packed[r]), r
+        q_row, rows64 = packed[1].copy(), packed[:64].copy()
+
+        def check_refdist(O, got=got, q_row=q_row, rows64=rows64):
+            for r in range(64):
+                assert int(got[r]) == O.distance_dense(q_row, rows64[r]), r
+
+        defer_check("reference distances dense (64 rows)", check_refdist)
         bytes_moved = n * bps + bps + 4 * n
         res["refdist_dense"] = {"rows": n, "bytes_per_row": bps, "kernel_ms": ms,
                                 "GB_per_s": bytes_moved / (ms / 1e3) / 1e9,
@@ -407,12 +436,16 @@ def run_wide(args, torch, capi, workloads, np):
     step()
     torch.cuda.synchronize()
     ms = tm.time(step, 2)
-    # spot check against the oracle's sparse route on a few rows
-    O = oracle()
+    # the first 400 rows' triangle for the oracle's sparse route
     chars = workloads.generate_chars(workloads.wide(n=400))
-    want = O.all_pairs_sparse(chars, True, 2, 65535)
-    host = d_out[: want.size * 2].cpu().numpy().view(np.uint16)
-    assert np.array_equal(host, want), "wide: triangle differs from the oracle"
+    head_cells = 400 * 399 // 2
+    host = d_out[: head_cells * 2].cpu().numpy().view(np.uint16).copy()
+
+    def check_wide(O, chars=chars, host=host):
+        want = O.all_pairs_sparse(chars, True, 2, 65535)
+        assert np.array_equal(host, want), "wide: triangle differs from the oracle"
+
+    defer_check("wide triangle, first 400 rows", check_wide)
     ops = cells * W * OPS_PER_WORD[P]
     res = {"n": n, "L": spec.L, "cells": cells, "semantics": "sparse_x", "planes": P,
            "out": "uint16", "ms_per_step": ms, "site_comparisons_per_s": cells * spec.L / (ms / 1e3),
@@ -423,14 +456,17 @@ def run_wide(args, torch, capi, workloads, np):
     from hammingdist_amd import _native
 
     ref = _native._synth_mutant(spec.L, 1012, 0.05, 0.0, 0.001, 0)
-    O_ref = O.pack(O.as_chars([ref]), True)[0]
-    q = torch.from_numpy(O_ref).to(dev)
+    q = torch.from_numpy(_native._pack([ref], True)).to(dev)
     d_rd = torch.empty(n, dtype=torch.int32, device=dev)
     rms = tm.time(lambda: capi.reference_distances_dev(q, d_packed, n, bps, None, n,
                                                        capi.HD_SEM_CHAR_X, d_rd, sh), 20)
     got = d_rd[:200].cpu().numpy().astype(np.uint32)
-    for r in range(200):
-        assert int(got[r]) == O.distance_char(ref, bytes(chars[r]).decode(), True), r
+
+    def check_wide_refdist(O, got=got, ref=ref, chars=chars):
+        for r in range(200):
+            assert int(got[r]) == O.distance_char(ref, bytes(chars[r]).decode(), True), r
+
+    defer_check("wide reference distances (200 rows)", check_wide_refdist)
     moved = n * bps + bps + 4 * n
     res["refdist_char_x"] = {"rows": n, "kernel_ms": rms, "GB_per_s": moved / (rms / 1e3) / 1e9,
                              "hbm_frac": moved / (rms / 1e3) / PEAK_HBM}
@@ -466,22 +502,23 @@ def run_batched(args, torch, capi, workloads, np):
     torch.cuda.synchronize()
     ms_batch = tm.time(lambda: b.run(sh), args.batch_reps)
     ms_seq = tm.time(lambda: b.run_sequential(sh), args.batch_reps)
-    # bit-exact check of a sample of jobs against the oracle
-    O = oracle()
+    # a sample of dense jobs for the oracle (sparse-route jobs are checked in
+    # tests/test_gpu_batch.py)
     rng = np.random.default_rng(1)
     b.run(sh)
     torch.cuda.synchronize()
+    sample = []
     for k in rng.choice(len(jobs), size=min(6, len(jobs)), replace=False):
         packed, sem, ob, md = packs[k]
-        if jobs[k][0] > 1500:
-            continue
-        got = b.output(int(k))
-        if sem == capi.HD_SEM_DENSE:
+        if jobs[k][0] <= 1500 and sem == capi.HD_SEM_DENSE:
+            sample.append((int(k), packed, ob, md, b.output(int(k))))
+
+    def check_batched(O, sample=sample):
+        for k, packed, ob, md, got in sample:
             want = O.all_pairs_dense(packed, ob, md)
-        else:
-            want = None  # sparse-route jobs are checked in tests/test_gpu_batch.py
-        if want is not None:
             assert np.array_equal(got, want), f"batched job {k} differs from the oracle"
+
+    defer_check(f"batched jobs ({len(sample)} dense)", check_batched)
     site_cmp = sum(n * (n - 1) // 2 * spec.L for (n, _, _), (spec, _) in zip(jobs, specs))
     res = {"datasets": len(jobs), "variant": variant,
            "autotune_ms": {str(k): v * 1e3 for k, v in times.items()},
@@ -518,10 +555,10 @@ def run_e2e(args, workloads, np):
         size = os.path.getsize(out)
         n = spec.n
         assert size == n * (n - 1), "every value is one digit: row i has 2i bytes"
-        # rows checked in place: row i starts at byte i(i-1)
-        O = oracle()
+        # sampled rows read in place (row i starts at byte i(i-1)) for the oracle
         packed = workloads.generate_packed(spec)
         rng = np.random.default_rng(2)
+        sample = []
         with open(out, "rb") as f:
             for i in [1, 2, 3, n - 1] + [int(x) for x in rng.integers(4, n - 1, 6)]:
                 f.seek(i * (i - 1))
@@ -529,7 +566,14 @@ def run_e2e(args, workloads, np):
                 assert line.endswith("\n")
                 vals = [int(v) for v in line[:-1].split(",")]
                 for j in rng.integers(0, i, 20):
-                    assert vals[int(j)] == O.distance_dense(packed[i], packed[int(j)], 2)
+                    sample.append((packed[i].copy(), packed[int(j)].copy(), vals[int(j)]))
+        del packed
+
+        def check_e2e(O, sample=sample):
+            for a, b, got in sample:
+                assert got == O.distance_dense(a, b, 2), "e2e: text differs from the oracle"
+
+        defer_check("end-to-end text rows (200 cells)", check_e2e)
         cells = n * (n - 1) // 2
         res = {"seconds": secs, "cells_per_s": cells / secs, "text_GB": size / 1e9,
                "reference_a100_seconds": 20.969, "speedup_vs_reference_a100": 20.969 / secs,
@@ -544,6 +588,110 @@ def run_e2e(args, workloads, np):
         os.rmdir(tmp)
 
 
+def run_api_paths(args, workloads, np):
+    """The reference's own API calls on FASTA files, timed end to end through
+    the package (host buffers, PCIe both ways):
+    - plain: from_fasta(remove_duplicates=True, max_distance=255) on n=2,000
+      sequences with 15 % duplicates, dump_lower_triangular and
+      dump_sequence_indices;
+    - wide: from_fasta_large(include_x=True) on n=100,000 x L=10,000 (the
+      5e9-cell uint16 triangle comes back to the host) and
+      fasta_reference_distances(include_x=True) of a fresh mutant against
+      all 100,000 records."""
+    import tempfile
+
+    import hammingdist_amd as hd
+    from hammingdist_amd import _native
+
+    out = {}
+    tmp = tempfile.mkdtemp(prefix="hd_api_")
+    files = []
+    try:
+        # plain
+        spec = workloads.plain()
+        fa = os.path.join(tmp, "plain.fa")
+        lt, idx = os.path.join(tmp, "plain_lt.txt"), os.path.join(tmp, "plain_idx.txt")
+        files += [fa, lt, idx]
+        workloads.write_fasta(spec, fa)
+        t = time.perf_counter()
+        ds = hd.from_fasta(fa, remove_duplicates=True, max_distance=255)
+        t_ds = time.perf_counter() - t
+        ds.dump_lower_triangular(lt)
+        ds.dump_sequence_indices(idx)
+        secs = time.perf_counter() - t
+        raw = open(fa, "rb").read()
+        got_idx = [int(v) for v in open(idx).read().split()]
+        got = np.array([ds[i, j] for i in range(1, 200) for j in range(i)], np.uint8)
+        n_unique = max(got_idx) + 1
+
+        def check_plain(O, raw=raw, got_idx=got_idx, got=got):
+            from oracle.fasta_oracle import read_fasta_bytes
+
+            data, want_idx = read_fasta_bytes(raw, True, 0)
+            assert got_idx == want_idx, "plain: sequence indices differ from the oracle"
+            chars = O.as_chars(data)
+            sub = chars[:200]
+            want = (O.all_pairs_sparse(sub, False, 1, 255) if O.use_sparse(chars, False)
+                    else O.all_pairs_dense(O.pack(sub), 1, 255))
+            assert np.array_equal(got, want), "plain: cells differ from the oracle"
+
+        defer_check("api plain: dedup indices and 200-row triangle", check_plain)
+        out["plain"] = {"n_in": spec.n, "n_unique": n_unique, "L": spec.L,
+                        "from_fasta_s": t_ds, "with_dumps_s": secs}
+        log(f"api plain: from_fasta {t_ds:.3f} s, with both dumps {secs:.3f} s "
+            f"({n_unique} unique of {spec.n})")
+        del ds
+        # wide
+        spec = workloads.wide()
+        fa = os.path.join(tmp, "wide.fa")
+        files.append(fa)
+        workloads.write_fasta(spec, fa)
+        t = time.perf_counter()
+        ds = hd.from_fasta_large(fa, include_x=True)
+        t_w = time.perf_counter() - t
+        n_sub = 300
+        head = fasta_head(fa, n_sub)
+        got = np.array([ds[i, j] for i in range(1, n_sub) for j in range(i)], np.uint16)
+        del ds
+        ref = _native._synth_mutant(spec.L, 1012, 0.05, 0.0, 0.001, 0)
+        t = time.perf_counter()
+        rd = hd.fasta_reference_distances(ref, fa, include_x=True)
+        t_r = time.perf_counter() - t
+        got_rd = rd[:200].tolist()
+
+        def check_wide_api(O, head=head, got=got, ref=ref, got_rd=got_rd):
+            want = O.all_pairs_sparse(O.as_chars(head), True, 2, 65535)
+            assert np.array_equal(got, want), "api wide: cells differ from the oracle"
+            assert got_rd == [O.distance_char(ref, s_, True) for s_ in head[:200]]
+
+        defer_check("api wide: 300-row triangle, 200 reference distances", check_wide_api)
+        out["wide"] = {"n": spec.n, "L": spec.L, "from_fasta_large_s": t_w,
+                       "cells": spec.n * (spec.n - 1) // 2,
+                       "fasta_reference_distances_s": t_r}
+        log(f"api wide: from_fasta_large {t_w:.2f} s, fasta_reference_distances {t_r:.3f} s")
+    finally:
+        for f in files:
+            if os.path.exists(f):
+                os.remove(f)
+        os.rmdir(tmp)
+    return out
+
+
+def fasta_head(path: str, k: int):
+    """The first k sequences of a FASTA file this benchmark wrote (one line
+    per record), read without loading the whole file."""
+    seqs = []
+    with open(path, "rb") as f:
+        f.readline()
+        for line in f:
+            if line.startswith(b">"):
+                continue
+            seqs.append(line.rstrip(b"\n").decode("latin-1"))
+            if len(seqs) == k:
+                break
+    return seqs
+
+
 def run_serving(args, torch, capi, workloads, np):
     from hammingdist_amd import serving
 
@@ -556,18 +704,28 @@ def run_serving(args, torch, capi, workloads, np):
     srv = capi.Server(db, device_bytes=24 << 30, host_bytes=4 << 30, streams=3, max_batch=32)
     log(f"serving: db resident in HBM after {time.perf_counter() - t:.1f}s")
     reqs = serving.make_requests(db, spec.L, args.serve_requests, seed=1004)
-    # bit-exact spot check: a few requests alone against the oracle
-    O = oracle()
+    # a few requests alone, sampled for the oracle
+    sample = []
     for r in reqs[:8]:
         got = serving.run_alone(srv, [r])[0]
         if r.kind == "query":
             idx = np.linspace(0, r.rows.size - 1, 50).astype(np.int64)
-            want = [O.distance_dense(r.query, db[int(r.rows[k])]) for k in idx]
-            assert got[idx].tolist() == want
+            rows = np.ascontiguousarray(db[r.rows[idx].astype(np.int64)])
+            sample.append(("query", r.query.copy(), rows, got[idx].copy()))
         else:
             sub = np.ascontiguousarray(db[r.rows[:200].astype(np.int64)])
-            want = O.all_pairs_dense(sub, 1, 255)
-            assert np.array_equal(got[: want.size], want)
+            sample.append(("all_pairs", None, sub, got[: 200 * 199 // 2].copy()))
+
+    def check_serving(O, sample=sample):
+        for kind, query, rows, got in sample:
+            if kind == "query":
+                want = [O.distance_dense(query, row) for row in rows]
+                assert got.tolist() == want, "serving: query differs from the oracle"
+            else:
+                want = O.all_pairs_dense(rows, 1, 255)
+                assert np.array_equal(got, want), "serving: all-pairs differs from the oracle"
+
+    defer_check("serving: 8 requests alone", check_serving)
     # capacity: everything submitted at once
     warm = reqs[: max(8, len(reqs) // 10)]
     serving.closed_loop(srv, warm)
@@ -619,6 +777,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
+    ap.add_argument("--no-api-paths", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only a 1 GiB device copy (counter calibration)")
     ap.add_argument("--headline-only", action="store_true",
@@ -628,6 +787,7 @@ def main() -> int:
         args.no_cpu_baseline = args.no_early_exit_run = args.no_refdist = True
         args.no_sparse = args.no_host_api = True
         args.no_wide = args.no_batched = args.no_serving = args.no_e2e = True
+        args.no_api_paths = True
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -688,10 +848,16 @@ def main() -> int:
         result["serving"] = run_serving(args, torch, capi, workloads, np)
     if not args.no_e2e:
         result["e2e_stream"] = run_e2e(args, workloads, np)
+    if not args.no_api_paths:
+        result["api_paths"] = run_api_paths(args, workloads, np)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(packed, args.L)
         log("cpu baseline done")
+        result["cpu_baseline"]["oracle_checks"] = run_checks()
+        log(f"oracle checks passed: {len(CHECKS)}")
+    else:
+        log(f"oracle checks skipped with the CPU baseline ({len(CHECKS)} recorded)")
     print(json.dumps(result), flush=True)
     return 0
 

@@ -9,6 +9,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine_common.h"
@@ -703,6 +704,57 @@ hipStream_t lib_stream() {
   return s;
 }
 
+// Copies `bytes` of device memory (ordered after the work on `s`) into
+// pageable host memory.  Large copies go through two pinned 64 MiB staging
+// buffers: the DMA of chunk c + 1 overlaps the copy of chunk c out of its
+// staging buffer, which several host threads share (they also take the
+// page faults of a freshly allocated destination in parallel).
+static void download(void *host, const void *dev, uint64_t bytes, hipStream_t s) {
+  constexpr uint64_t kStage = 64ull << 20;
+  if (bytes <= kStage) {
+    HD_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s));
+    HD_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  static std::mutex mu; // one large download at a time shares the staging
+  std::lock_guard<std::mutex> lock(mu);
+  // Held until the process ends (never freed, so no HIP call runs during
+  // static destruction).
+  static PinnedBuffer *stage = nullptr;
+  static Event *done = nullptr;
+  if (!stage) {
+    auto *st = new PinnedBuffer[2];
+    st[0].alloc(kStage);
+    st[1].alloc(kStage);
+    done = new Event[2];
+    stage = st;
+  }
+  const uint64_t chunks = (bytes + kStage - 1) / kStage;
+  const unsigned threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  auto enqueue = [&](uint64_t c) {
+    const uint64_t off = c * kStage, len = std::min(kStage, bytes - off);
+    HD_HIP(hipMemcpyAsync(stage[c & 1].get(), static_cast<const uint8_t *>(dev) + off, len,
+                          hipMemcpyDeviceToHost, s));
+    HD_HIP(hipEventRecord(done[c & 1].get(), s));
+  };
+  enqueue(0);
+  for (uint64_t c = 0; c < chunks; ++c) {
+    if (c + 1 < chunks) enqueue(c + 1); // its buffer was emptied in step c - 1
+    HD_HIP(hipEventSynchronize(done[c & 1].get()));
+    const uint64_t off = c * kStage, len = std::min(kStage, bytes - off);
+    const uint8_t *src = stage[c & 1].get<uint8_t>();
+    uint8_t *dst = static_cast<uint8_t *>(host) + off;
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < threads; ++t)
+      pool.emplace_back([=] {
+        const uint64_t b = len * t / threads, e = len * (t + 1) / threads;
+        std::memcpy(dst + b, src + b, e - b);
+      });
+    std::memcpy(dst, src, len / threads);
+    for (auto &th : pool) th.join();
+  }
+}
+
 // Uploads n packed rows and builds their planes (plus the nibble mask).
 struct Encoded {
   Geometry g;
@@ -837,9 +889,7 @@ int hd_all_pairs(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
     const uint64_t TI = tile_rows(v);
     run_allpairs(e.g, e.planes.get<uint32_t>(), out_bytes, max_dist, true, v, 0,
                  (n + TI - 1) / TI, d_out.get(), 0, s);
-    HD_HIP(hipMemcpyAsync(host_out, d_out.get(), cells * out_bytes,
-                          hipMemcpyDeviceToHost, s));
-    HD_HIP(hipStreamSynchronize(s));
+    download(host_out, d_out.get(), cells * out_bytes, s);
   });
 }
 

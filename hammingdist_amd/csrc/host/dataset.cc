@@ -75,10 +75,10 @@ Route choose_route(const Packer &pack, std::uint64_t n, std::uint64_t L, bool in
 }
 
 template <typename T>
-std::vector<T> distances_packed(const Packer &pack, std::uint64_t n, std::uint64_t L,
+Cells<T> distances_packed(const Packer &pack, std::uint64_t n, std::uint64_t L,
                                 bool include_x, bool use_gpu, int max_distance,
                                 const std::function<void()> &after_pack) {
-  std::vector<T> result(n < 2 ? 0 : n * (n - 1) / 2, 0);
+  Cells<T> result(n < 2 ? 0 : n * (n - 1) / 2); // every cell is written below
   const T max_dist = safe_int_cast<T>(max_distance);
   Timer timer;
   const std::uint64_t bps = bytes_per_seq(L);
@@ -97,7 +97,7 @@ std::vector<T> distances_packed(const Packer &pack, std::uint64_t n, std::uint64
 }
 
 template <typename T>
-std::vector<T> distances(std::vector<std::string> &data, bool include_x,
+Cells<T> distances(std::vector<std::string> &data, bool include_x,
                          bool clear_input_data, bool use_gpu, int max_distance) {
   const Packer pack = [&](bool x, std::uint64_t bps) { return pack_all(data, x, bps); };
   return distances_packed<T>(pack, data.size(), common_length(data), include_x, use_gpu,
@@ -106,10 +106,10 @@ std::vector<T> distances(std::vector<std::string> &data, bool include_x,
                              });
 }
 
-template std::vector<std::uint8_t> distances<std::uint8_t>(std::vector<std::string> &, bool,
-                                                           bool, bool, int);
-template std::vector<std::uint16_t> distances<std::uint16_t>(std::vector<std::string> &,
-                                                             bool, bool, bool, int);
+template Cells<std::uint8_t> distances<std::uint8_t>(std::vector<std::string> &, bool, bool,
+                                                     bool, int);
+template Cells<std::uint16_t> distances<std::uint16_t>(std::vector<std::string> &, bool,
+                                                       bool, bool, int);
 
 // ---- DataSet ---------------------------------------------------------------
 
@@ -127,7 +127,7 @@ template <typename T> DataSet<T>::DataSet(const std::string &filename) {
   std::ifstream count_stream(filename);
   nsamples = std::count(std::istreambuf_iterator<char>(count_stream),
                         std::istreambuf_iterator<char>(), '\n');
-  result.resize(nsamples * (nsamples + 1) / 2);
+  result.resize(nsamples * (nsamples + 1) / 2, T{0});
   std::ifstream stream(filename);
   std::string line;
   std::size_t i = 0, current = 0;
@@ -143,7 +143,7 @@ template <typename T> DataSet<T>::DataSet(const std::string &filename) {
 }
 
 // include/hamming/hamming.hh:57-61
-template <typename T> DataSet<T>::DataSet(std::vector<T> &&distances) : result{std::move(distances)} {
+template <typename T> DataSet<T>::DataSet(Cells<T> &&distances) : result{std::move(distances)} {
   const auto x = 8 * result.size() + 1;
   const auto r = static_cast<std::size_t>(std::round(std::sqrt(static_cast<double>(x))));
   nsamples = (r + 1) / 2;
@@ -206,7 +206,7 @@ DataSet<DefaultDistIntType> from_csv(const std::string &filename) {
 
 // include/hamming/hamming.hh:152-166
 template <typename T> DataSet<T> from_lower_triangular(const std::string &filename) {
-  std::vector<T> distances;
+  Cells<T> distances;
   std::ifstream stream(filename);
   std::string line;
   while (std::getline(stream, line)) {

@@ -35,6 +35,8 @@ template <typename T> struct NoInitAllocator : std::allocator<T> {
   }
 };
 using PackedRows = std::vector<GeneBlock, NoInitAllocator<GeneBlock>>;
+// A lower triangle of distances; its cells are written by the engine in full.
+template <typename T> using Cells = std::vector<T, NoInitAllocator<T>>;
 
 // ---- encoding (src/hamming_impl.cc:23-44, 244-267) ------------------------
 std::array<GeneBlock, 256> lookup_table(bool include_x = false);
@@ -134,13 +136,13 @@ Route choose_route(const Packer &pack, std::uint64_t n, std::uint64_t L, bool in
 // distances<T> on rows a Packer provides (n rows of L sites); after_pack
 // runs once the rows are packed.
 template <typename T>
-std::vector<T> distances_packed(const Packer &pack, std::uint64_t n, std::uint64_t L,
+Cells<T> distances_packed(const Packer &pack, std::uint64_t n, std::uint64_t L,
                                 bool include_x, bool use_gpu, int max_distance,
                                 const std::function<void()> &after_pack);
 
 // distances<T>, include/hamming/hamming_impl.hh:68-176
 template <typename T>
-std::vector<T> distances(std::vector<std::string> &data, bool include_x,
+Cells<T> distances(std::vector<std::string> &data, bool include_x,
                          bool clear_input_data, bool use_gpu, int max_distance);
 
 // ---- result container (include/hamming/hamming.hh:22-143) -----------------
@@ -151,7 +153,7 @@ template <typename T> struct DataSet {
                    int max_distance = std::numeric_limits<int>::max());
   // Full-matrix CSV reader (from_csv), include/hamming/hamming.hh:34-55.
   explicit DataSet(const std::string &filename);
-  explicit DataSet(std::vector<T> &&distances);
+  explicit DataSet(Cells<T> &&distances);
 
   void dump(const std::string &filename) const;
   void dump_lower_triangular(const std::string &filename) const;
@@ -160,7 +162,7 @@ template <typename T> struct DataSet {
   int get(std::size_t i, std::size_t j) const;
 
   std::size_t nsamples = 0;
-  std::vector<T> result;
+  Cells<T> result;
   std::vector<std::size_t> sequence_indices{};
 };
 
@@ -210,7 +212,7 @@ template <typename T>
 void write_rows(const std::string &filename, const T *cells, std::uint64_t row_begin,
                 std::uint64_t row_end, bool append);
 template <typename T>
-void write_lower_triangular(const std::string &filename, const std::vector<T> &cells);
+void write_lower_triangular(const std::string &filename, const Cells<T> &cells);
 
 // ---- small parallel helpers -----------------------------------------------
 unsigned worker_count();

@@ -185,7 +185,7 @@ template void write_rows<std::uint16_t>(const std::string &, const std::uint16_t
 // (no file) for an empty vector; a cell count that is not triangular ends in
 // an incomplete row terminated by ',' as partial_write_lower_triangular does.
 template <typename T>
-void write_lower_triangular(const std::string &filename, const std::vector<T> &cells) {
+void write_lower_triangular(const std::string &filename, const Cells<T> &cells) {
   const std::uint64_t size = cells.size();
   if (size == 0) return;
   std::uint64_t rows = 1; // full rows 1..rows-1
@@ -208,9 +208,9 @@ void write_lower_triangular(const std::string &filename, const std::vector<T> &c
 }
 
 template void write_lower_triangular<std::uint8_t>(const std::string &,
-                                                   const std::vector<std::uint8_t> &);
+                                                   const Cells<std::uint8_t> &);
 template void write_lower_triangular<std::uint16_t>(const std::string &,
-                                                    const std::vector<std::uint16_t> &);
+                                                    const Cells<std::uint16_t> &);
 
 // DataSet::dump_sparse, include/hamming/hamming.hh:79-114: "i j d\n" for each
 // cell with d <= threshold, in row-major order.  The cells are selected and

@@ -55,6 +55,11 @@ def read_fasta(path: str, remove_duplicates: bool = False, n: int = 0):
     """(data, sequence_indices) as read_fasta returns them."""
     with open(path, "rb") as f:
         raw = f.read()
+    return read_fasta_bytes(raw, remove_duplicates, n)
+
+
+def read_fasta_bytes(raw: bytes, remove_duplicates: bool = False, n: int = 0):
+    """read_fasta on the bytes of a file."""
     data, indices, index = [], [], {}
     for seq in _records(raw, n):
         if remove_duplicates:

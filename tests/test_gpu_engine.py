@@ -249,3 +249,16 @@ def test_sparse_emit_row_major(out_bytes, rng):
             np.testing.assert_array_equal(trip[:, 0], I[sel])
             np.testing.assert_array_equal(trip[:, 1], J[sel])
             np.testing.assert_array_equal(trip[:, 2], cells[sel])
+
+
+@pytest.mark.parametrize("out_bytes", [1, 2])
+def test_large_triangle_download_in_chunks(out_bytes, rng):
+    # 12,000 rows: 72e6 cells (69 MB as uint8, 137 MB as uint16) come back
+    # to the host through the chunked staging copy (64 MiB chunks).
+    n, L = 12_000, 60
+    seqs = random_strings(rng, n, L, "ACGT-N")
+    packed = O.pack(O.as_chars(seqs))
+    md = 255 if out_bytes == 1 else 65535
+    got = engine_all_pairs(packed, capi.HD_SEM_DENSE, out_bytes, md)
+    want = O.all_pairs_dense(packed, out_bytes, md)
+    np.testing.assert_array_equal(got, want)
