@@ -27,7 +27,7 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // One dataset of a batch, as the kernels see it.
 struct BatchJobDev {
   const uint32_t *planes;
-  const uint32_t *corr; // dense 3-op code: Z[n_pad], G[n_pad]
+  const uint32_t *corr; // correction counts (codes with has_corr), else null
   void *out;
   uint64_t tile_begin; // first index of this job in the batch's tile list
   uint32_t n, n_pad, W, P, out_bytes, max_dist, check_every;
@@ -36,11 +36,12 @@ struct BatchJobDev {
 struct EncodeJobDev {
   const uint8_t *packed;
   uint32_t *planes;
-  uint32_t *corr;       // dense 3-op code: Z[n_pad], G[n_pad]
+  uint32_t *corr;       // correction counts (codes with has_corr), else null
   uint64_t bpr;
   uint64_t block_begin; // first encode block of this job
   uint64_t count_begin; // first count block (row group) of this job
   uint32_t n, n_pad, W, P, runs, mode;
+  uint32_t op; // plane code (kernels.h), for the correction counts
   Table16 tbl;
 };
 
@@ -80,12 +81,14 @@ __global__ __launch_bounds__(256) void k_batch_encode(const EncodeJobDev *__rest
              (int)J.mode);
 }
 
-// Z and G counts of every dense 3-op job: one workgroup per row group.
+// Correction counts of every job whose code has them: one workgroup per row
+// group.
 __global__ __launch_bounds__(256) void k_batch_counts(const EncodeJobDev *__restrict__ jobs,
                                                       uint32_t n_jobs) {
   const uint32_t j = find_job(jobs, n_jobs, blockIdx.x, &EncodeJobDev::count_begin);
   const EncodeJobDev &J = jobs[j];
-  dense3_counts_group(J.planes, J.W, (uint32_t)(blockIdx.x - J.count_begin), J.n_pad, J.corr);
+  corr_counts_group(J.planes, J.W, (uint32_t)(blockIdx.x - J.count_begin), J.n_pad,
+                    (int)J.op, J.corr);
 }
 
 __global__ void k_reset_counter(unsigned long long *counter) { *counter = 0; }
@@ -129,8 +132,11 @@ __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_allpairs(
     case 4:
       HD_BATCH_TILE(4, C, j0)
       break;
-    case kClass2:
-      HD_BATCH_TILE(kClass2, C, j0)
+    case kGap2:
+      HD_BATCH_TILE(kGap2, C, j0)
+      break;
+    case kGap3:
+      HD_BATCH_TILE(kGap3, C, j0)
       break;
     case kClass3:
       HD_BATCH_TILE(kClass3, C, j0)
@@ -167,9 +173,12 @@ __global__ __launch_bounds__(kWave *kBatchWaves) void k_batch_grid(
   } else if (P == 4) {
     tile_wave<4, R, C>(J.planes, J.n, J.W, i0, j0, J.out, J.out_bytes, 0, J.max_dist,
                        J.check_every, nullptr, J.n_pad);
-  } else if (P == kClass2) {
-    tile_wave<kClass2, R, C>(J.planes, J.n, J.W, i0, j0, J.out, J.out_bytes, 0, J.max_dist,
-                             J.check_every, nullptr, J.n_pad);
+  } else if (P == kGap2) {
+    tile_wave<kGap2, R, C>(J.planes, J.n, J.W, i0, j0, J.out, J.out_bytes, 0, J.max_dist,
+                           J.check_every, J.corr, J.n_pad);
+  } else if (P == kGap3) {
+    tile_wave<kGap3, R, C>(J.planes, J.n, J.W, i0, j0, J.out, J.out_bytes, 0, J.max_dist,
+                           J.check_every, J.corr, J.n_pad);
   } else if (P == kClass3) {
     tile_wave<kClass3, R, C>(J.planes, J.n, J.W, i0, j0, J.out, J.out_bytes, 0, J.max_dist,
                              J.check_every, nullptr, J.n_pad);
@@ -325,7 +334,7 @@ void prepare(hd_batch *b, int variant, int early_exit) {
     EncodeJobDev e{};
     e.packed = J.packed.get<uint8_t>();
     e.planes = J.planes.get<uint32_t>();
-    e.corr = is_dense3(J.g.code) ? e.planes + J.g.plane_words() : nullptr;
+    e.corr = has_corr(J.g.code.op) ? e.planes + J.g.plane_words() : nullptr;
     e.bpr = J.bpr;
     e.block_begin = blocks;
     e.count_begin = count_blocks;
@@ -333,11 +342,12 @@ void prepare(hd_batch *b, int variant, int early_exit) {
     e.n_pad = (uint32_t)J.g.n_pad;
     e.W = J.g.W;
     e.P = (uint32_t)J.g.code.planes;
+    e.op = (uint32_t)J.g.code.op;
     e.runs = (J.g.W + kEncodeRun - 1) / kEncodeRun;
     std::memcpy(e.tbl.v, J.g.code.table, 16);
     e.mode = (uint32_t)encode_mode(J.g.code);
     blocks += (uint64_t)(e.n_pad / 256) * e.runs;
-    if (is_dense3(J.g.code)) count_blocks += e.n_pad / 64;
+    if (has_corr(J.g.code.op)) count_blocks += e.n_pad / 64;
     enc.push_back(e);
     if (J.n < 2) continue;
     BatchJobDev d{};

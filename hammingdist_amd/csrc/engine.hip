@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -64,8 +65,11 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
   case HD_SEM_SPARSE_X: {
     // f = classes differ and neither is '-'.  Classes: the letters, X
     // (SPARSE_X only) and one invalid class for every other code but '-'.
-    // Each class present gets a code 0, 1, ...; 2 code bits for up to 4
-    // classes, else 3; '-' sets the plane after the code bits.
+    // Each class present gets a code 0, 1, ...; with the kGap codes '-'
+    // gets the next one, and 2 code bits serve up to 4 codes, else 3; the
+    // plane after the code bits marks '-'.  Rows of 2^28 sites or more
+    // (beyond the kGap counters' range) take kClass3, whose '-' plane masks
+    // each pair.
     const bool x_class = semantics == HD_SEM_SPARSE_X;
     const uint32_t known = kLetters | kGapBit | (x_class ? kXBit : 0u);
     int next = 0;
@@ -75,11 +79,16 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
     for (int l = 0; l < 4; ++l)
       code[one_hot[l]] = (mask >> one_hot[l]) & 1u ? (uint8_t)next++ : 0;
     const int x_code = x_class && (mask & kXBit) ? next++ : 0;
-    const int bits = next <= 4 ? 2 : 3;
-    pc.op = bits == 2 ? kClass2 : kClass3;
+    // HD_SPARSE_CLASS_CODE=1 selects kClass3 for any row length (it lets
+    // the long-row code be exercised on small data).
+    const char *force_class = std::getenv("HD_SPARSE_CLASS_CODE");
+    const bool gap_code = 32ull * words < (1ull << 28) && !(force_class && *force_class == '1');
+    const int dash = gap_code ? next++ : 0;
+    const int bits = gap_code && next <= 4 ? 2 : 3;
+    pc.op = gap_code ? (bits == 2 ? kGap2 : kGap3) : kClass3;
     pc.planes = bits + 1;
     for (int v = 0; v < 16; ++v) {
-      if (v == 15) pc.table[v] = (uint8_t)(1u << bits);
+      if (v == 15) pc.table[v] = (uint8_t)(dash | (1u << bits));
       else if ((kLetters >> v) & 1u) pc.table[v] = code[v];
       else if (x_class && v == 3) pc.table[v] = (uint8_t)x_code;
       else pc.table[v] = (uint8_t)(inv_code < 0 ? 0 : inv_code);
@@ -132,20 +141,23 @@ static void launch_allpairs_t(const uint32_t *planes, uint32_t n,
                               uint32_t max_dist, uint32_t check_every,
                               hipStream_t s) {
   const uint32_t *corr =
-      P == kDense3 ? planes + (size_t)W * stored_planes(P) * n_pad : nullptr;
+      has_corr(P) ? planes + (size_t)W * stored_planes(P) * n_pad : nullptr;
   constexpr uint64_t M = (64 * R) / (NW * C);
   const uint64_t t0 = tiles_before(rb0, M), t1 = tiles_before(rb1, M);
   const uint64_t nt = t1 - t0;
   if (nt == 0) return;
   if (nt > 0x7fffffffull) throw Error(HD_ERR_INVALID, "too many tiles in one launch");
-  if (check_every < W)
-    hipLaunchKernelGGL((k_allpairs<P, R, C, NW, OutT, true>), dim3((uint32_t)nt),
-                       dim3(64 * NW), 0, s, planes, n, W, t0, (uint32_t)nt,
-                       out, cell_offset, max_dist, check_every, corr, n_pad);
+  using Kernel = void (*)(const uint32_t *, uint32_t, uint32_t, uint64_t, uint32_t, OutT *,
+                         uint64_t, uint32_t, uint32_t, const uint32_t *, uint32_t);
+  Kernel kernel;
+  if constexpr (P == kGap2 || P == kGap3)
+    kernel = check_every < W ? k_allpairs_gap<P, R, C, NW, OutT, true>
+                             : k_allpairs_gap<P, R, C, NW, OutT, false>;
   else
-    hipLaunchKernelGGL((k_allpairs<P, R, C, NW, OutT, false>), dim3((uint32_t)nt),
-                       dim3(64 * NW), 0, s, planes, n, W, t0, (uint32_t)nt,
-                       out, cell_offset, max_dist, check_every, corr, n_pad);
+    kernel = check_every < W ? k_allpairs<P, R, C, NW, OutT, true>
+                             : k_allpairs<P, R, C, NW, OutT, false>;
+  hipLaunchKernelGGL(kernel, dim3((uint32_t)nt), dim3(64 * NW), 0, s, planes, n, W, t0,
+                     (uint32_t)nt, out, cell_offset, max_dist, check_every, corr, n_pad);
   HD_HIP(hipGetLastError());
 }
 
@@ -188,9 +200,12 @@ static void launch_allpairs_o(int P, int v, const uint32_t *planes, uint32_t n,
   case 5:
     return launch_allpairs_p<5, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
                                       cell_offset, max_dist, check_every, s);
-  case kClass2:
-    return launch_allpairs_p<kClass2, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
-                                            cell_offset, max_dist, check_every, s);
+  case kGap2:
+    return launch_allpairs_p<kGap2, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
+                                          cell_offset, max_dist, check_every, s);
+  case kGap3:
+    return launch_allpairs_p<kGap3, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
+                                          cell_offset, max_dist, check_every, s);
   case kClass3:
     return launch_allpairs_p<kClass3, OutT>(v, planes, n, n_pad, W, rb0, rb1, out,
                                             cell_offset, max_dist, check_every, s);
@@ -278,11 +293,12 @@ __global__ __launch_bounds__(256) void k_encode(const uint8_t *__restrict__ pack
   encode_run(packed, n, bpr, W, P, s_tbl, planes, i, blockIdx.y * kEncodeRun, mode);
 }
 
-// Z and G counts of the dense 3-op code, one workgroup per row group.
-__global__ __launch_bounds__(256) void k_dense3_counts(const uint32_t *__restrict__ planes,
-                                                       uint32_t W, uint32_t n_pad,
-                                                       uint32_t *__restrict__ corr) {
-  dense3_counts_group(planes, W, blockIdx.x, n_pad, corr);
+// Correction counts of the kDense3 and kGap codes, one workgroup per row
+// group.
+__global__ __launch_bounds__(256) void k_corr_counts(const uint32_t *__restrict__ planes,
+                                                     uint32_t W, uint32_t n_pad, int P,
+                                                     uint32_t *__restrict__ corr) {
+  corr_counts_group(planes, W, blockIdx.x, n_pad, P, corr);
 }
 
 int encode_mode(const PlaneCode &c) {
@@ -307,9 +323,10 @@ void launch_encode(const Geometry &g, const uint8_t *d_packed, uint32_t *d_plane
                      g.bpr, (uint32_t)g.n_pad, g.W, g.code.planes, t, encode_mode(g.code),
                      d_planes);
   HD_HIP(hipGetLastError());
-  if (is_dense3(g.code)) {
-    hipLaunchKernelGGL(k_dense3_counts, dim3((uint32_t)(g.n_pad / 64)), dim3(256), 0, s,
-                       d_planes, g.W, (uint32_t)g.n_pad, d_planes + g.plane_words());
+  if (has_corr(g.code.op)) {
+    hipLaunchKernelGGL(k_corr_counts, dim3((uint32_t)(g.n_pad / 64)), dim3(256), 0, s,
+                       d_planes, g.W, (uint32_t)g.n_pad, g.code.op,
+                       d_planes + g.plane_words());
     HD_HIP(hipGetLastError());
   }
 }

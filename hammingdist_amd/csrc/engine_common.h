@@ -10,6 +10,7 @@
 #include <string>
 
 #include "hammingdist_amd.h"
+#include "kernels.h"
 
 namespace hd {
 
@@ -135,7 +136,9 @@ private:
 // function (its template parameter P, kernels.h): op = planes (4, 5) counts a
 // site when no plane is set in both sequences; op = 3 is the dense 3-op code
 // over 4 planes whose rows also carry per-row correction counts after the
-// planes; op = 7, 8 are the class codes (2 or 3 code bits and a '-' plane).
+// planes; op = 7, 9 are the sparse codes with '-' as a code (2 or 3 code
+// bits and a '-' plane, correction counts after the planes); op = 8 is the
+// sparse class code of very long rows (3 code bits and a '-' plane).
 struct PlaneCode {
   int planes = 4;
   int op = 4;
@@ -158,11 +161,12 @@ struct Geometry {
   uint32_t W = 0; // 32-site words per row
   PlaneCode code;
   uint64_t plane_words() const { return (uint64_t)W * code.planes * n_pad; }
-  // Planes, then (dense 3-op code) the per-row correction counts: whole-row
-  // Z and G, then per 32-word chunk (kernels.h, dense3_corr_words).
+  // Planes, then (codes with correction counts) two whole-row count slots
+  // and two per-32-word-chunk slots (kernels.h, dense3_corr_words and
+  // gap_corr_words).
   uint64_t total_words() const {
     return plane_words() +
-           (is_dense3(code) ? (2 + 2 * (uint64_t)((W + 31) / 32)) * n_pad : 0);
+           (has_corr(code.op) ? (2 + 2 * (uint64_t)((W + 31) / 32)) * n_pad : 0);
   }
 };
 Geometry geometry(uint64_t n, uint64_t bpr, int semantics, uint32_t mask);

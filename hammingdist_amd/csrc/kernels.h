@@ -59,15 +59,27 @@ struct Table16 {
 //                early exit on, the counts of each kCheckWords-word chunk are
 //                added after that chunk, so every vote sees the exact partial
 //                distance; both come from dense3_counts.
-//   P = kClass2, kClass3  a site counts when the classes differ and neither
-//                is '-' (the sparse semantics).  Every class present gets a
-//                2- or 3-bit code; '-' sets the last plane.  Per pair and 32
-//                sites: one XOR, one (x ^ y) | s per further code bit, one
-//                s & ~g_i & ~g_j and one popcount-accumulate (4 or 5 ops).
+//   P = kGap2, kGap3  the sparse semantics: a site counts when the classes
+//                differ and neither is '-'.  Every class present and '-'
+//                get distinct 2- or 3-bit codes (planes 0 .. B-1), plane B
+//                marks the '-' sites.  Per pair and 32 sites: one XOR, one
+//                (x ^ y) | s per further code bit and one popcount-accumulate
+//                (3 or 4 ops), f = codes differ.  Per site f - [row i '-'] -
+//                [row j '-'] + 2 [both '-'] is the site's distance term, so
+//                popcount(f) - G_i - G_j + 2 |g_i & g_j| is the distance,
+//                where G counts the '-' sites of a row (padding included).
+//                The last term is summed over the words where a column has
+//                a '-' only: a per-row bitmap of such words (gap_corr_words)
+//                drives a wave-uniform loop after the main one.  With the
+//                early exit on, chunk counts of G make the votes use
+//                popcount - G_i - G_j, a lower bound of the partial distance.
+//   P = kClass3  the sparse semantics for rows of 2^28 sites or more: 3 code
+//                bits and a '-' plane, s & ~g_i & ~g_j per pair (5 ops).
 // ---------------------------------------------------------------------------
 constexpr int kDense3 = 3;
-constexpr int kClass2 = 7;
+constexpr int kGap2 = 7;
 constexpr int kClass3 = 8;
+constexpr int kGap3 = 9;
 
 // Words between two early-exit votes (1024 sites).
 constexpr uint32_t kCheckWords = 32;
@@ -79,9 +91,22 @@ __host__ __device__ constexpr uint64_t dense3_corr_words(uint32_t W, uint64_t n_
   return (2 + 2 * (uint64_t)((W + kCheckWords - 1) / kCheckWords)) * n_pad;
 }
 
+// kGap2 / kGap3 correction words after the planes (same size as kDense3's):
+// G[n_pad] (whole rows, twice), then Gc[T][n_pad] per chunk of kCheckWords
+// words and BM[T][n_pad], bit k of BM[t][row] set when word 32 t + k of the
+// row holds a '-'.
+__host__ __device__ constexpr uint64_t gap_corr_words(uint32_t W, uint64_t n_pad) {
+  return dense3_corr_words(W, n_pad);
+}
+
 // Planes stored per row and word for plane code P.
 __host__ __device__ constexpr int stored_planes(int P) {
-  return P == kDense3 ? 4 : P == kClass2 ? 3 : P == kClass3 ? 4 : P;
+  return P == kDense3 ? 4 : P == kGap2 ? 3 : P == kClass3 ? 4 : P == kGap3 ? 4 : P;
+}
+
+// True for the codes whose rows carry correction counts after the planes.
+__host__ __device__ constexpr bool has_corr(int P) {
+  return P == kDense3 || P == kGap2 || P == kGap3;
 }
 
 // Tile t -> (row block b, column block cb) of a triangle tiled with M column
@@ -98,9 +123,10 @@ __device__ __forceinline__ void tile_coords(uint64_t t, uint32_t M, uint32_t &b,
 // One wave's share of a tile: rows i0 + 64 q + lane (q < R) against columns
 // j0 .. j0 + C - 1, stored into the triangle at cell i(i-1)/2 + j - cell_offset
 // as out_bytes-wide cells (1, 2 or 4; a constant where the caller's is).
-// corr (kDense3 only): the correction counts of the planes' rows
-// (dense3_corr_words); check_every is W or kCheckWords.  EE = false
-// compiles the loop without early-exit votes (check_every is then ignored).
+// corr (codes with has_corr only): the correction counts of the planes' rows
+// (dense3_corr_words / gap_corr_words); check_every is W or kCheckWords.
+// EE = false compiles the loop without early-exit votes (check_every is then
+// ignored).
 template <int P, int R, int C, bool EE = true>
 __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, uint32_t n,
                                           uint32_t W, uint32_t i0, uint32_t j0,
@@ -110,18 +136,22 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
                                           const uint32_t *__restrict__ corr,
                                           uint32_t n_pad) {
   constexpr bool D3 = P == kDense3;
-  constexpr bool CLS = P == kClass2 || P == kClass3;
-  constexpr int PS = stored_planes(P); // planes per row and word
-  constexpr int PB = D3 ? 3 : PS;      // column planes used
-  constexpr int B0 = D3 ? 1 : 0;       // first column plane used
-  // kDense3 accumulators carry OFF + popcounts + Z - G counts added so far.
-  // Rows have fewer than 2^28 sites (the code is only chosen then), so
-  // every partial sum stays within [OFF - 2^28, OFF + 2^29], the cap can be
-  // clamped to 2^28 and pairs outside the triangle start at kD3Idle, which
-  // stays above OFF + md and below 2^32.
-  constexpr uint32_t OFF = D3 ? 0x80000000u : 0u;
+  constexpr bool GAP = P == kGap2 || P == kGap3;
+  constexpr bool CLS = P == kClass3;
+  constexpr bool OFFSET = D3 || GAP;     // accumulators carry OFF + counts
+  constexpr int PS = stored_planes(P);   // planes per row and word
+  constexpr int GB = P == kGap2 ? 2 : 3; // kGap*: code bits; plane GB = '-'
+  constexpr int PA = GAP ? GB : PS;      // row planes used by the main loop
+  constexpr int PB = D3 ? 3 : PA;        // column planes used
+  constexpr int B0 = D3 ? 1 : 0;         // first column plane used
+  // Offset codes: accumulators carry OFF + popcounts + the row and column
+  // counts added so far.  Rows have fewer than 2^28 sites (the codes are
+  // only chosen then), so every partial sum of a pair stays within
+  // [OFF - 2^29, OFF + 3 * 2^28], the cap can be clamped to 2^28 and pairs
+  // outside the triangle start at kD3Idle, which stays above OFF + md.
+  constexpr uint32_t OFF = OFFSET ? 0x80000000u : 0u;
   constexpr uint32_t kD3Idle = 0xD0000000u;
-  const uint32_t md = D3 ? min(max_dist, 0x10000000u) : max_dist;
+  const uint32_t md = OFFSET ? min(max_dist, 0x10000000u) : max_dist;
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t i_end = min(i0 + (uint32_t)(kWave * R), n);
   if (j0 + 1 >= i_end) return; // no pair j < i inside this wave's block
@@ -129,10 +159,10 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   // Pairs outside the triangle start saturated so that the early-exit vote
   // only waits for real pairs; they are never stored.
   uint32_t acc[R][C];
-  // Adds row counts zc[row] - gc[column] to every accumulator (kDense3).
-  // The column counts are read into VGPRs (the offset `vzero` is an opaque
-  // per-lane 0), which keeps them out of the SGPR file the column planes
-  // fill.
+  // Adds row counts zc[row] - gc[column] (kDense3) or -zc[row] - gc[column]
+  // (kGap*) to every accumulator.  The column counts are read into VGPRs
+  // (the offset `vzero` is an opaque per-lane 0), which keeps them out of
+  // the SGPR file the column planes fill.
   auto add_counts = [&](const uint32_t *__restrict__ zc, const uint32_t *__restrict__ gc) {
     uint32_t vzero;
     __asm__ volatile("v_mov_b32 %0, 0" : "=v"(vzero));
@@ -141,26 +171,28 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
     for (int c = 0; c < C; ++c) gj[c] = gc[j0 + c + vzero];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const uint32_t zi = zc[min(i0 + kWave * q + lane, n_pad - 1)];
+      const uint32_t z = zc[min(i0 + kWave * q + lane, n_pad - 1)];
+      const uint32_t zi = GAP ? 0u - z : z;
 #pragma unroll
       for (int c = 0; c < C; ++c) acc[q][c] += zi - gj[c];
     }
   };
-  if constexpr (D3 && EE) {
+  if constexpr (OFFSET && EE) {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const uint32_t i = i0 + kWave * q + lane;
 #pragma unroll
       for (int c = 0; c < C; ++c) acc[q][c] = (j0 + c < i && i < n) ? OFF : kD3Idle;
     }
-  } else if constexpr (D3) {
+  } else if constexpr (OFFSET) {
     // No votes: the whole-row counts go in at the start.
     uint32_t gj[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) gj[c] = corr[n_pad + j0 + c];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const uint32_t zi = corr[min(i0 + kWave * q + lane, n_pad - 1)];
+      const uint32_t z = corr[min(i0 + kWave * q + lane, n_pad - 1)];
+      const uint32_t zi = GAP ? 0u - z : z;
 #pragma unroll
       for (int c = 0; c < C; ++c) acc[q][c] = OFF + zi - gj[c];
     }
@@ -180,17 +212,17 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   // constant address space lets the wave-uniform column words be scalar
   // loads even where the pointer itself came from memory.
   using cptr = const __attribute__((address_space(4))) uint32_t *;
-  cptr pa = (cptr)(planes + (size_t)(i0 / kWave) * group + lane);
-  cptr pb = (cptr)(planes + (size_t)(j0 / kWave) * group + (j0 % kWave));
+  const cptr pb0 = (cptr)(planes + (size_t)(j0 / kWave) * group + (j0 % kWave));
+  cptr pa = (cptr)(planes + (size_t)(i0 / kWave) * group + lane), pb = pb0;
   uint32_t k = 0, chunk = 0;
   while (k < W) {
     const uint32_t k_end = EE ? min(W, k + check_every) : W;
     for (; k < k_end; ++k) {
-      uint32_t a[R][PS];
+      uint32_t a[R][PA];
 #pragma unroll
       for (int q = 0; q < R; ++q)
 #pragma unroll
-        for (int p = 0; p < PS; ++p) a[q][p] = pa[q * group + p * kWave];
+        for (int p = 0; p < PA; ++p) a[q][p] = pa[q * group + p * kWave];
       uint32_t bw[C][PB];
 #pragma unroll
       for (int p = 0; p < PB; ++p)
@@ -207,6 +239,12 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
             s = __builtin_amdgcn_bitop3_b32(a[q][1], bw[c][0], a[q][0], 0xBE); // z|(x^y)
             s = __builtin_amdgcn_bitop3_b32(a[q][2], bw[c][1], s, 0x41);
             s = __builtin_amdgcn_bitop3_b32(a[q][3], bw[c][2], s, 0x71);
+          } else if constexpr (GAP) {
+            // codes differ: XOR, then 0xBE = (x ^ y) | z per further bit
+            s = a[q][0] ^ bw[c][0];
+#pragma unroll
+            for (int p = 1; p < GB; ++p)
+              s = __builtin_amdgcn_bitop3_b32(a[q][p], bw[c][p], s, 0xBE);
           } else if constexpr (CLS) {
             // code bits differ (0xBE = (x^y)|z), then z & ~x & ~y (0x02)
             // with the two '-' planes
@@ -229,14 +267,14 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
       pa += PS * kWave;
       pb += PS * kWave;
     }
-    if constexpr (D3 && EE) {
+    if constexpr (OFFSET && EE) {
       if (check_every < W) {
         // The barrier keeps the count loads here, after the chunk, instead
         // of holding them in registers across it.
         __asm__ volatile("" ::: "memory");
         const uint64_t T = (W + kCheckWords - 1) / kCheckWords;
-        add_counts(corr + (2 + chunk) * (uint64_t)n_pad,
-                   corr + (2 + T + chunk) * (uint64_t)n_pad);
+        const uint32_t *zc = corr + (2 + chunk) * (uint64_t)n_pad;
+        add_counts(zc, D3 ? corr + (2 + T + chunk) * (uint64_t)n_pad : zc);
       }
     }
     ++chunk;
@@ -249,10 +287,49 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
       if (__all(lo >= OFF + md)) break;
     }
   }
-  if constexpr (D3 && EE) {
+  if constexpr (OFFSET && EE) {
     if (check_every >= W) { // one chunk: the whole-row counts
       __asm__ volatile("" ::: "memory");
       add_counts(corr, corr + n_pad);
+    }
+  }
+  if constexpr (GAP) {
+    // + 2 |g_i & g_j| over the words where one of the wave's columns has a
+    // '-': the union of the columns' bitmap words, one wave-uniform step per
+    // word.  After an early exit every real pair's count is already a lower
+    // bound >= md, and adding to it keeps it there.
+    __asm__ volatile("" ::: "memory");
+    const uint32_t T = (W + kCheckWords - 1) / kCheckWords;
+    const cptr bm = (cptr)(corr + (2 + (uint64_t)T) * n_pad);
+    // The rows' base address is recomputed from a lane id the compiler
+    // cannot tie to `lane`, so no copy of it stays live across the main loop.
+    const uint32_t lane_b = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    const cptr pr = (cptr)(planes + (size_t)(i0 / kWave) * group + lane_b);
+    for (uint32_t t = 0; t < T; ++t) {
+      uint32_t u = 0;
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+        if (j0 + c < n) u |= bm[(uint64_t)t * n_pad + j0 + c];
+      while (u) {
+        const uint32_t w = t * kCheckWords + (uint32_t)__builtin_ctz(u);
+        u &= u - 1;
+        const uint64_t step = ((uint64_t)w * PS + GB) * kWave;
+        uint32_t ga[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) ga[q] = pr[q * group + step];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const uint32_t gb = pb0[step + c];
+          if (gb != 0) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+              const uint32_t x = ga[q] & gb;
+              acc[q][c] += __builtin_popcount(x);
+              acc[q][c] += __builtin_popcount(x);
+            }
+          }
+        }
+      }
     }
   }
 
@@ -274,8 +351,9 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   }
 }
 
+// Tile t of a launch over tiles [tile_begin, tile_begin + n_tiles).
 template <int P, int R, int C, int NW, typename OutT, bool EE>
-__global__ __launch_bounds__(kWave *NW) void k_allpairs(
+__device__ __forceinline__ void allpairs_tile(
     const uint32_t *__restrict__ planes, uint32_t n, uint32_t W, uint64_t tile_begin,
     uint32_t n_tiles, OutT *__restrict__ out, uint64_t cell_offset, uint32_t max_dist,
     uint32_t check_every, const uint32_t *__restrict__ corr, uint32_t n_pad) {
@@ -294,6 +372,27 @@ __global__ __launch_bounds__(kWave *NW) void k_allpairs(
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   tile_wave<P, R, C, EE>(planes, n, W, b * TI, cb * TJ + wave * C, out, sizeof(OutT),
                          cell_offset, max_dist, check_every, corr, n_pad);
+}
+
+template <int P, int R, int C, int NW, typename OutT, bool EE>
+__global__ __launch_bounds__(kWave *NW) void k_allpairs(
+    const uint32_t *__restrict__ planes, uint32_t n, uint32_t W, uint64_t tile_begin,
+    uint32_t n_tiles, OutT *__restrict__ out, uint64_t cell_offset, uint32_t max_dist,
+    uint32_t check_every, const uint32_t *__restrict__ corr, uint32_t n_pad) {
+  allpairs_tile<P, R, C, NW, OutT, EE>(planes, n, W, tile_begin, n_tiles, out, cell_offset,
+                                       max_dist, check_every, corr, n_pad);
+}
+
+// The kGap codes' kernel, held to 4 waves per SIMD (at most 128 VGPRs):
+// left free, the compiler spends registers on the tile loop's schedule and
+// runs 3 waves per SIMD, which measured slower.
+template <int P, int R, int C, int NW, typename OutT, bool EE>
+__global__ __launch_bounds__(kWave *NW, 4) void k_allpairs_gap(
+    const uint32_t *__restrict__ planes, uint32_t n, uint32_t W, uint64_t tile_begin,
+    uint32_t n_tiles, OutT *__restrict__ out, uint64_t cell_offset, uint32_t max_dist,
+    uint32_t check_every, const uint32_t *__restrict__ corr, uint32_t n_pad) {
+  allpairs_tile<P, R, C, NW, OutT, EE>(planes, n, W, tile_begin, n_tiles, out, cell_offset,
+                                       max_dist, check_every, corr, n_pad);
 }
 
 // ---------------------------------------------------------------------------
@@ -400,32 +499,46 @@ __device__ __forceinline__ void encode_run(const uint8_t *__restrict__ packed, u
   }
 }
 
-// Per-row counts of the kDense3 code for row group g (64 rows): Z = invalid
-// sites (all four planes clear), G = '-' sites (plane 0), per chunk of
-// kCheckWords words and over the whole row (layout: dense3_corr_words).  One
-// 256-thread workgroup per group: wave w counts the chunks t = w mod 4 for
-// the 64 rows (lane = row, 256-byte loads), then wave 0 adds the four
-// partial row totals.
-__device__ __forceinline__ void dense3_counts_group(const uint32_t *__restrict__ planes,
-                                                    uint32_t W, uint32_t g, uint32_t n_pad,
-                                                    uint32_t *__restrict__ corr) {
+// Per-row correction counts of row group g (64 rows), layout
+// dense3_corr_words / gap_corr_words.  kDense3: Z = invalid sites (all four
+// planes clear) and G = '-' sites (plane 0).  kGap2 / kGap3: G = '-' sites
+// (plane B) in both whole-row slots, per chunk G and the bitmap of the
+// chunk's words that hold a '-'.  Per chunk of kCheckWords words and over
+// the whole row.  One 256-thread workgroup per group: wave w counts the
+// chunks t = w mod 4 for the 64 rows (lane = row, 256-byte loads), then
+// wave 0 adds the four partial row totals.
+__device__ __forceinline__ void corr_counts_group(const uint32_t *__restrict__ planes,
+                                                  uint32_t W, uint32_t g, uint32_t n_pad,
+                                                  int P, uint32_t *__restrict__ corr) {
   __shared__ uint32_t s_z[4][kWave], s_g[4][kWave];
   const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const uint32_t row = g * kWave + lane;
   const uint32_t T = (W + kCheckWords - 1) / kCheckWords;
-  const uint32_t *base = planes + (size_t)g * W * 4 * kWave + lane;
+  const bool d3 = P == kDense3;
+  const uint32_t PS = (uint32_t)stored_planes(P);
+  const uint32_t gp = P == kGap2 ? 2 : 3; // '-' plane of the kGap codes
+  const uint32_t *base = planes + (size_t)g * W * PS * kWave + lane;
   uint32_t z = 0, gc = 0;
   for (uint32_t t = wave; t < T; t += 4) {
     uint32_t zt = 0, gt = 0;
-    const uint32_t k_end = min(W, (t + 1) * kCheckWords);
-    for (uint32_t k = t * kCheckWords; k < k_end; ++k) {
-      const uint32_t *w = base + (size_t)k * 4 * kWave;
-      const uint32_t p0 = w[0], p1 = w[kWave], p2 = w[2 * kWave], p3 = w[3 * kWave];
-      zt += __builtin_popcount(~(p0 | p1 | p2 | p3));
-      gt += __builtin_popcount(p0);
+    const uint32_t k0 = t * kCheckWords, k_end = min(W, k0 + kCheckWords);
+    if (d3) {
+      for (uint32_t k = k0; k < k_end; ++k) {
+        const uint32_t *w = base + (size_t)k * 4 * kWave;
+        const uint32_t p0 = w[0], p1 = w[kWave], p2 = w[2 * kWave], p3 = w[3 * kWave];
+        zt += __builtin_popcount(~(p0 | p1 | p2 | p3));
+        gt += __builtin_popcount(p0);
+      }
+    } else {
+      // zt: bitmap of the words with a '-'
+      for (uint32_t k = k0; k < k_end; ++k) {
+        const uint32_t gw = base[((size_t)k * PS + gp) * kWave];
+        gt += __builtin_popcount(gw);
+        zt |= (gw != 0 ? 1u : 0u) << (k - k0);
+      }
     }
-    corr[(2 + (uint64_t)t) * n_pad + row] = zt;
-    corr[(2 + (uint64_t)T + t) * n_pad + row] = gt;
+    corr[(2 + (uint64_t)t) * n_pad + row] = d3 ? zt : gt;
+    corr[(2 + (uint64_t)T + t) * n_pad + row] = d3 ? gt : zt;
     z += zt;
     gc += gt;
   }
@@ -433,8 +546,10 @@ __device__ __forceinline__ void dense3_counts_group(const uint32_t *__restrict__
   s_g[wave][lane] = gc;
   __syncthreads();
   if (wave == 0) {
-    corr[row] = s_z[0][lane] + s_z[1][lane] + s_z[2][lane] + s_z[3][lane];
-    corr[n_pad + row] = s_g[0][lane] + s_g[1][lane] + s_g[2][lane] + s_g[3][lane];
+    const uint32_t zs = s_z[0][lane] + s_z[1][lane] + s_z[2][lane] + s_z[3][lane];
+    const uint32_t gs = s_g[0][lane] + s_g[1][lane] + s_g[2][lane] + s_g[3][lane];
+    corr[row] = d3 ? zs : gs;
+    corr[n_pad + row] = gs;
   }
 }
 

@@ -63,14 +63,17 @@ def test_invalid_arguments_are_rejected_before_device_use():
     # planes layout is pure host arithmetic
     n_pad, W, P, total = capi.planes_layout(1000, 14952, capi.HD_SEM_DENSE, 0xFFFF)
     assert (n_pad, W, P) == (1024, 935, 4) and total == 1024 * 935 * 4
-    # sparse semantics: a 2-bit class code for up to 4 classes present, else
-    # 3 bits, plus the '-' plane
+    # sparse semantics: every class present and '-' get a code, 2 bits for up
+    # to 4 codes, else 3, plus the '-' plane and the correction counts
     letters_gap = (1 << 1) | (1 << 2) | (1 << 4) | (1 << 8) | (1 << 15)
-    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE, letters_gap)[2] == 3
-    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE, letters_gap | 1)[2] == 4
+    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE, letters_gap)[2] == 4
+    assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE, (1 << 1) | (1 << 2) | (1 << 4))[2] == 3
     assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, 0xFFFF)[2] == 4
     assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, (1 << 3) | (1 << 0))[2] == 3
     assert capi.planes_layout(10, 8, capi.HD_SEM_SPARSE_X, (1 << 1) | (1 << 15))[2] == 3
+    n_pad, W, P, total = capi.planes_layout(1000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF)
+    # (whole-row counts twice, per 32-word chunk counts and bitmaps: 2 + 2 * 10)
+    assert (n_pad, W, P) == (1024, 313, 4) and total == 1024 * 313 * 4 + 22 * 1024
     assert capi.planes_layout(10, 8, capi.HD_SEM_CHAR_X, 1 << 3)[2] == 5
     # dense data holding only A C G T '-' and invalid codes: the 3-op code,
     # 4 planes plus per-row counts; any other code (X = 3): nibble planes
@@ -80,3 +83,17 @@ def test_invalid_arguments_are_rejected_before_device_use():
     assert (n_pad, W, P) == (1024, 935, 4) and total == 1024 * 935 * 4 + 62 * 1024
     assert capi.planes_layout(1000, 14952, capi.HD_SEM_DENSE,
                               acgt_gap_inv | (1 << 3))[3] == 1024 * 935 * 4
+
+
+def test_long_row_sparse_code_layout(monkeypatch):
+    from hammingdist_amd import capi
+
+    # Rows of 2^28 sites or more take the class code: planes only, no counts
+    # (bytes_per_seq 2^27 = 2^28 sites = 2^23 words).
+    n_pad, W, P, total = capi.planes_layout(2, 1 << 27, capi.HD_SEM_SPARSE, 0xFFFF)
+    assert (n_pad, W, P) == (512, 1 << 23, 4) and total == 512 * (1 << 23) * 4
+    n_pad, W, P, total = capi.planes_layout(2, (1 << 27) - 16, capi.HD_SEM_SPARSE, 0xFFFF)
+    assert total > 512 * W * 4  # one word shorter: the gap code and its counts
+    monkeypatch.setenv("HD_SPARSE_CLASS_CODE", "1")
+    n_pad, W, P, total = capi.planes_layout(1000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF)
+    assert (n_pad, W, P) == (1024, 313, 4) and total == 1024 * 313 * 4

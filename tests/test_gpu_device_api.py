@@ -55,6 +55,98 @@ def test_every_variant_sparse_x(variant, rng):
     np.testing.assert_array_equal(got, want)
 
 
+def gap_heavy_rows(rng, n, L, alphabet):
+    """Rows over `alphabet` with '-' in runs (up to a third of the row), some
+    rows all '-', some with a '-' in every 32-site word, some without any."""
+    seqs = []
+    for r in range(n):
+        s = rng.choice(list(alphabet), size=L)
+        kind = r % 7
+        if kind == 0:
+            s[:] = "-"
+        elif kind == 1:
+            s[rng.integers(0, 32, size=1)[0]::32] = "-"
+        elif kind in (2, 3, 4):
+            for _ in range(int(rng.integers(1, 4))):
+                a = int(rng.integers(0, L))
+                s[a:a + int(rng.integers(1, max(2, L // 3)))] = "-"
+        seqs.append("".join(s))
+    return seqs
+
+
+GAP_CASES = [(600, 1290, "ACGTNX"), (300, 2100, "ACG"), (70, 33, "AC")]
+GAP_CAPS = [(2, 65535), (1, 3), (4, 2**32 - 1), (2, 40), (1, 0)]
+_gap_oracle = {}
+
+
+def gap_case(case, include_x):
+    """Rows and oracle triangles of one gap case, shared by every variant."""
+    key = (case, include_x)
+    if key not in _gap_oracle:
+        n, L, alphabet = GAP_CASES[case]
+        chars = O.as_chars(gap_heavy_rows(np.random.default_rng(2024 + case), n, L, alphabet))
+        want = {cap: O.all_pairs_sparse(chars, include_x, *cap) for cap in GAP_CAPS}
+        _gap_oracle[key] = (O.pack(chars, include_x), want)
+    return _gap_oracle[key]
+
+
+@pytest.mark.parametrize("variant", range(5))
+@pytest.mark.parametrize("include_x", [False, True])
+def test_gap_code_every_variant_with_gap_runs(variant, include_x):
+    # The sparse semantics' '-' correction: per-row '-' counts, the bitmap of
+    # words holding a '-' and 2 |g_i & g_j| over them; L = 1290 leaves 16
+    # padding sites in the last word, L = 2100 spans three early-exit chunks.
+    sem = capi.HD_SEM_SPARSE_X if include_x else capi.HD_SEM_SPARSE
+    for case, (n, L, _) in enumerate(GAP_CASES):
+        packed, wants = gap_case(case, include_x)
+        for out_bytes, md in GAP_CAPS:
+            want = wants[(out_bytes, md)]
+            for early in (False, True):
+                got = run_dev(packed, sem, out_bytes, md, early, variant)
+                np.testing.assert_array_equal(
+                    got, want, err_msg=f"v={variant} n={n} L={L} md={md} early={early}")
+
+
+@pytest.mark.parametrize("variant", [0, 2, 4])
+def test_long_row_class_code(variant, monkeypatch):
+    # kClass3 serves rows of 2^28 sites or more; HD_SPARSE_CLASS_CODE=1
+    # selects it for these small rows.
+    monkeypatch.setenv("HD_SPARSE_CLASS_CODE", "1")
+    for include_x in (False, True):
+        sem = capi.HD_SEM_SPARSE_X if include_x else capi.HD_SEM_SPARSE
+        for case in range(len(GAP_CASES)):
+            packed, wants = gap_case(case, include_x)
+            n, bps = packed.shape
+            assert capi.planes_layout(n, bps, sem, 0xFFFF)[3] == \
+                capi.planes_layout(n, bps, sem, 0xFFFF)[0] * ((2 * bps + 31) // 32) * 4
+            for (out_bytes, md) in [(2, 65535), (1, 3)]:
+                for early in (False, True):
+                    got = run_dev(packed, sem, out_bytes, md, early, variant)
+                    np.testing.assert_array_equal(got, wants[(out_bytes, md)],
+                                                  err_msg=f"case={case} md={md}")
+
+
+def test_gap_code_early_exit_waits_for_shared_gaps(rng):
+    # Rows share long '-' runs in the first chunks, so popcount - G_i - G_j
+    # runs far below the partial distance there, and differ only later: the
+    # votes use that lower bound, so no tile may stop before its distances
+    # reach the cap.
+    n, L = 500, 5000
+    root = rng.choice(list("ACGT"), size=L)
+    seqs = []
+    for _ in range(n):
+        s = root.copy()
+        s[: int(rng.integers(1500, 2500))] = "-"
+        s[int(rng.integers(2600, L - 10)) + rng.integers(0, 10, size=int(rng.integers(0, 6)))] = "A"
+        seqs.append("".join(s))
+    chars = O.as_chars(seqs)
+    packed = O.pack(chars)
+    for md in (0, 1, 2, 4, 9):
+        want = O.all_pairs_sparse(chars, False, 1, md)
+        got = run_dev(packed, capi.HD_SEM_SPARSE, 1, md, True, 0)
+        np.testing.assert_array_equal(got, want, err_msg=f"md={md}")
+
+
 def test_early_exit_identical_on_saturating_data():
     # Every pair far above max_dist: the early exit fires in the first chunk.
     rng = np.random.default_rng(7)

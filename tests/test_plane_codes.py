@@ -89,7 +89,8 @@ def class_code_table(present, bits):
                                      ("inv", "A", "C", "G", "T", "X"),
                                      ("inv", "X")])
 def test_class_code_chain_is_the_sparse_rule(present):
-    bits = 2 if len(present) <= 4 else 3
+    # kClass3, the code of rows of 2^28 sites or more: always 3 code bits
+    bits = 3
     table = class_code_table(present, bits)
     classes = list(present) + ["-"]
     for a, b in itertools.product(classes, classes):
@@ -111,3 +112,38 @@ def test_one_hot_chain_is_the_dense_rule_on_nibbles():
             s = bitop3(a[p], b[p], s, 0xEA)
         s = bitop3(a[3], b[3], s, 0x15)
         assert s == int((x & y) == 0)
+
+
+def gap_code_table(present):
+    """kGap2 / kGap3: codes 0, 1, ... for the classes present (invalid, A, C,
+    G, T, X) and then '-'; 2 code bits for up to 4 codes, else 3; the plane
+    after the code bits marks '-'."""
+    order = [c for c in ("inv", "A", "C", "G", "T", "X") if c in present] + ["-"]
+    bits = 2 if len(order) <= 4 else 3
+    table = {c: [(k >> b) & 1 for b in range(bits)] + [int(c == "-")]
+             for k, c in enumerate(order)}
+    return table, bits
+
+
+@pytest.mark.parametrize("present", [("A", "C", "G"),
+                                     ("A", "C", "G", "T"),
+                                     ("inv", "A", "C", "G", "T"),
+                                     ("A", "C", "G", "T", "X"),
+                                     ("inv", "A", "C", "G", "T", "X"),
+                                     ("inv", "X")])
+def test_gap_code_chain_with_corrections_is_the_sparse_rule(present):
+    # Per site: f = codes differ (XOR, then (x ^ y) | s per further bit);
+    # f - [row '-'] - [column '-'] + 2 [both '-'] is the sparse rule, and
+    # without the last term it never exceeds it (the early-exit votes use
+    # that lower bound).
+    table, bits = gap_code_table(present)
+    assert len(set(tuple(v[:bits]) for v in table.values())) == len(table)
+    classes = list(present) + ["-"]
+    for a, b in itertools.product(classes, classes):
+        pa, pb = table[a], table[b]
+        s = pa[0] ^ pb[0]
+        for p in range(1, bits):
+            s = bitop3(pa[p], pb[p], s, 0xBE)
+        g_a, g_b = pa[bits], pb[bits]
+        assert s - g_a - g_b + 2 * (g_a & g_b) == sparse_rule(a, b), (present, a, b)
+        assert s - g_a - g_b <= sparse_rule(a, b)
