@@ -787,6 +787,8 @@ def main() -> int:
                     help="only a 1 GiB device copy (counter calibration)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline step (for profiling)")
+    ap.add_argument("--wide-only", action="store_true",
+                    help="only the wide leg (for profiling)")
     args = ap.parse_args()
     if args.headline_only:
         args.no_cpu_baseline = args.no_early_exit_run = args.no_refdist = True
@@ -820,6 +822,13 @@ def main() -> int:
     import torch
 
     from hammingdist_amd import capi, workloads
+
+    if args.wide_only:
+        wide = run_wide(args, torch, capi, workloads, np)
+        sys.stdout.flush()
+        with os.fdopen(json_fd, "w") as out:
+            out.write(json.dumps({"wide": wide}) + "\n")
+        return 0
 
     ceiling = valu_ceiling()
     if ceiling:
