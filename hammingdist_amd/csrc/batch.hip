@@ -486,7 +486,7 @@ int hd_batch_run_sequential(hd_batch *b, void *stream) {
       if (J.n == 0) continue;
       launch_encode(J.g, J.packed.get<uint8_t>(), J.planes.get<uint32_t>(), s);
       if (J.n < 2) continue;
-      const int v = default_variant(J.g.code.planes, J.n);
+      const int v = default_variant(J.g, b->early_exit != 0, J.max_dist);
       const uint64_t TI = 64ull * variant_of(v).R;
       run_allpairs(J.g, J.planes.get<uint32_t>(), J.out_bytes, J.max_dist, b->early_exit != 0,
                    v, 0, (J.n + TI - 1) / TI, J.out.get(), 0, s);

@@ -47,8 +47,7 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
   switch (semantics) {
   case HD_SEM_DENSE:
     pc.planes = 4;
-    if ((mask & ~(kLetters | kGapBit | kInvalidBit) & 0xFFFFu) == 0 &&
-        32ull * words < (1ull << 28)) {
+    if ((mask & ~(kLetters | kGapBit | kInvalidBit) & 0xFFFFu) == 0 && words < kBufWords) {
       // Only 0, A, C, G, T and '-': the 3-op code, planes (g, c1, c2, c3).
       pc.op = kDense3;
       pc.table[1] = 0xE;  // A: c1 c2 c3
@@ -67,9 +66,8 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
     // (SPARSE_X only) and one invalid class for every other code but '-'.
     // Each class present gets a code 0, 1, ...; with the kGap codes '-'
     // gets the next one, and 2 code bits serve up to 4 codes, else 3; the
-    // plane after the code bits marks '-'.  Rows of 2^28 sites or more
-    // (beyond the kGap counters' range) take kClass3, whose '-' plane masks
-    // each pair.
+    // plane after the code bits marks '-'.  Rows of kBufWords words or more
+    // take kClass3, whose '-' plane masks each pair.
     const bool x_class = semantics == HD_SEM_SPARSE_X;
     const uint32_t known = kLetters | kGapBit | (x_class ? kXBit : 0u);
     int next = 0;
@@ -82,7 +80,7 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
     // HD_SPARSE_CLASS_CODE=1 selects kClass3 for any row length (it lets
     // the long-row code be exercised on small data).
     const char *force_class = std::getenv("HD_SPARSE_CLASS_CODE");
-    const bool gap_code = 32ull * words < (1ull << 28) && !(force_class && *force_class == '1');
+    const bool gap_code = words < kBufWords && !(force_class && *force_class == '1');
     const int dash = gap_code ? next++ : 0;
     const int bits = gap_code && next <= 4 ? 2 : 3;
     pc.op = gap_code ? (bits == 2 ? kGap2 : kGap3) : kClass3;
@@ -128,9 +126,14 @@ static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 int variant_count() { return kNumVariants; }
 const Variant &variant_of(int v) { return kVariants[v]; }
 
-int default_variant(int planes, uint64_t n) {
+int default_variant(const Geometry &g, bool early_exit, uint32_t max_dist) {
+  const uint64_t n = g.n;
   if (n < 512) return 4;
-  if (planes <= 4) return n < 2048 ? 3 : 0;
+  // With early-exit votes the kGap codes' kernel runs faster with 8 columns
+  // per wavefront (fewer SGPRs to hold across each vote).
+  const bool votes = check_interval(early_exit, max_dist, g.W) < g.W;
+  if (votes && (g.code.op == kGap2 || g.code.op == kGap3)) return 2;
+  if (g.code.planes <= 4) return n < 2048 ? 3 : 0;
   return 2;
 }
 
@@ -881,7 +884,7 @@ int hd_all_pairs_dev(const uint32_t *d_planes, uint64_t n, uint64_t bytes_per_se
     check_max_dist(out_bytes, max_dist);
     const Geometry g = geometry(n, bytes_per_seq, semantics, nibble_mask);
     if (n < 2) return;
-    const int v = variant < 0 ? default_variant(g.code.planes, n) : variant;
+    const int v = variant < 0 ? default_variant(g, early_exit != 0, max_dist) : variant;
     HD_REQUIRE(v < kNumVariants, "unknown variant");
     const uint64_t TI = tile_rows(v);
     const uint64_t nb = (n + TI - 1) / TI;
@@ -902,7 +905,7 @@ int hd_all_pairs(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
     upload_encode(e, packed, n, bytes_per_seq, semantics, s);
     const uint64_t cells = tri_cells(n);
     DeviceBuffer d_out(cells * out_bytes);
-    const int v = default_variant(e.g.code.planes, n);
+    const int v = default_variant(e.g, true, max_dist);
     const uint64_t TI = tile_rows(v);
     run_allpairs(e.g, e.planes.get<uint32_t>(), out_bytes, max_dist, true, v, 0,
                  (n + TI - 1) / TI, d_out.get(), 0, s);
@@ -923,7 +926,7 @@ int hd_all_pairs_stream(const uint8_t *packed, uint64_t n, uint64_t bytes_per_se
     Stream copy;
     Encoded e;
     upload_encode(e, packed, n, bytes_per_seq, semantics, s);
-    const int v = default_variant(e.g.code.planes, n);
+    const int v = default_variant(e.g, true, max_dist);
     const uint64_t TI = tile_rows(v);
     const uint64_t nb = (n + TI - 1) / TI;
     if (band_cells == 0) band_cells = 64ull << 20;

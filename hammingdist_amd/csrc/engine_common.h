@@ -178,7 +178,9 @@ struct Variant {
 };
 int variant_count();
 const Variant &variant_of(int v);
-int default_variant(int planes, uint64_t n);
+// The tile variant run_allpairs uses for geometry g (its n rows) when the
+// caller names none.
+int default_variant(const Geometry &g, bool early_exit, uint32_t max_dist);
 
 void launch_encode(const Geometry &g, const uint8_t *d_packed, uint32_t *d_planes,
                    hipStream_t s);

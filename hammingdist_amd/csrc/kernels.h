@@ -73,13 +73,22 @@ struct Table16 {
 //                drives a wave-uniform loop after the main one.  With the
 //                early exit on, chunk counts of G make the votes use
 //                popcount - G_i - G_j, a lower bound of the partial distance.
-//   P = kClass3  the sparse semantics for rows of 2^28 sites or more: 3 code
-//                bits and a '-' plane, s & ~g_i & ~g_j per pair (5 ops).
+//   P = kClass3  the sparse semantics for rows of kBufWords words or more: 3
+//                code bits and a '-' plane, s & ~g_i & ~g_j per pair (5 ops).
 // ---------------------------------------------------------------------------
 constexpr int kDense3 = 3;
 constexpr int kGap2 = 7;
 constexpr int kClass3 = 8;
 constexpr int kGap3 = 9;
+
+// The offset codes (kDense3, kGap2, kGap3) serve rows of fewer than
+// kBufWords 32-site words (8,388,608 sites): their tile loop addresses the
+// rows of a tile (at most 8 row groups of 4 planes) with 31-bit buffer
+// offsets.  Longer rows take the nibble and class codes.
+constexpr uint32_t kBufWords = 1u << 18;
+// Word 3 of a raw buffer resource on gfx9-family GPUs (no format conversion,
+// 32-bit data, stride 0).
+constexpr int kBufferRsrcWord3 = 0x00020000;
 
 // Words between two early-exit votes (1024 sites).
 constexpr uint32_t kCheckWords = 32;
@@ -145,10 +154,10 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   constexpr int PB = D3 ? 3 : PA;        // column planes used
   constexpr int B0 = D3 ? 1 : 0;         // first column plane used
   // Offset codes: accumulators carry OFF + popcounts + the row and column
-  // counts added so far.  Rows have fewer than 2^28 sites (the codes are
-  // only chosen then), so every partial sum of a pair stays within
-  // [OFF - 2^29, OFF + 3 * 2^28], the cap can be clamped to 2^28 and pairs
-  // outside the triangle start at kD3Idle, which stays above OFF + md.
+  // counts added so far.  Rows have fewer than 2^23 sites (kBufWords; the
+  // codes are only chosen then), so every partial sum of a pair stays
+  // within [OFF - 2^24, OFF + 3 * 2^23], the cap can be clamped to 2^28 and
+  // pairs outside the triangle start at kD3Idle, which stays above OFF + md.
   constexpr uint32_t OFF = OFFSET ? 0x80000000u : 0u;
   constexpr uint32_t kD3Idle = 0xD0000000u;
   const uint32_t md = OFFSET ? min(max_dist, 0x10000000u) : max_dist;
@@ -214,15 +223,29 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   using cptr = const __attribute__((address_space(4))) uint32_t *;
   const cptr pb0 = (cptr)(planes + (size_t)(j0 / kWave) * group + (j0 % kWave));
   cptr pa = (cptr)(planes + (size_t)(i0 / kWave) * group + lane), pb = pb0;
+  // The offset codes (rows of fewer than kBufWords words) read the rows
+  // through one buffer resource at the tile's first row group: the per-lane
+  // offsets stay fixed, the step's offset is a scalar, so no VALU op goes
+  // into addresses.  Every offset is below R * group * 4 < 2^31 bytes.
+  constexpr bool BUF = OFFSET;
+  const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(planes + (size_t)(i0 / kWave) * group), (short)0, -1, kBufferRsrcWord3);
+  int row_off[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) row_off[q] = (int)((lane + (uint32_t)(q * group)) * 4);
   uint32_t k = 0, chunk = 0;
   while (k < W) {
     const uint32_t k_end = EE ? min(W, k + check_every) : W;
     for (; k < k_end; ++k) {
       uint32_t a[R][PA];
+      const int step_off = (int)(k * (uint32_t)(PS * kWave * 4));
 #pragma unroll
       for (int q = 0; q < R; ++q)
 #pragma unroll
-        for (int p = 0; p < PA; ++p) a[q][p] = pa[q * group + p * kWave];
+        for (int p = 0; p < PA; ++p)
+          a[q][p] = BUF ? __builtin_amdgcn_raw_buffer_load_b32(rows_rsrc, row_off[q],
+                                                               step_off + p * kWave * 4, 0)
+                        : pa[q * group + p * kWave];
       uint32_t bw[C][PB];
 #pragma unroll
       for (int p = 0; p < PB; ++p)
@@ -294,10 +317,10 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
     }
   }
   if constexpr (GAP) {
-    // + 2 |g_i & g_j| over the words where one of the wave's columns has a
-    // '-': the union of the columns' bitmap words, one wave-uniform step per
-    // word.  After an early exit every real pair's count is already a lower
-    // bound >= md, and adding to it keeps it there.
+    // + 2 |g_i & g_j| over the words where column j has a '-': per column, a
+    // wave-uniform loop over the set bits of its bitmap.  After an early exit
+    // every real pair's count is already a lower bound >= md, and adding to
+    // it keeps it there.
     __asm__ volatile("" ::: "memory");
     const uint32_t T = (W + kCheckWords - 1) / kCheckWords;
     const cptr bm = (cptr)(corr + (2 + (uint64_t)T) * n_pad);
@@ -305,28 +328,20 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
     // cannot tie to `lane`, so no copy of it stays live across the main loop.
     const uint32_t lane_b = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
     const cptr pr = (cptr)(planes + (size_t)(i0 / kWave) * group + lane_b);
-    for (uint32_t t = 0; t < T; ++t) {
-      uint32_t u = 0;
 #pragma unroll
-      for (int c = 0; c < C; ++c)
-        if (j0 + c < n) u |= bm[(uint64_t)t * n_pad + j0 + c];
-      while (u) {
-        const uint32_t w = t * kCheckWords + (uint32_t)__builtin_ctz(u);
-        u &= u - 1;
-        const uint64_t step = ((uint64_t)w * PS + GB) * kWave;
-        uint32_t ga[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) ga[q] = pr[q * group + step];
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
+    for (int c = 0; c < C; ++c) {
+      for (uint32_t t = 0; t < (j0 + c < n ? T : 0u); ++t) {
+        uint32_t u = bm[(uint64_t)t * n_pad + j0 + c];
+        while (u) {
+          const uint32_t w = t * kCheckWords + (uint32_t)__builtin_ctz(u);
+          u &= u - 1;
+          const uint64_t step = ((uint64_t)w * PS + GB) * kWave;
           const uint32_t gb = pb0[step + c];
-          if (gb != 0) {
 #pragma unroll
-            for (int q = 0; q < R; ++q) {
-              const uint32_t x = ga[q] & gb;
-              acc[q][c] += __builtin_popcount(x);
-              acc[q][c] += __builtin_popcount(x);
-            }
+          for (int q = 0; q < R; ++q) {
+            const uint32_t x = pr[q * group + step] & gb;
+            acc[q][c] += __builtin_popcount(x);
+            acc[q][c] += __builtin_popcount(x);
           }
         }
       }

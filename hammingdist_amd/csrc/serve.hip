@@ -296,7 +296,7 @@ struct hd_server {
         launch_encode(r.g, dbase + r.lay.d_packed,
                       reinterpret_cast<uint32_t *>(dbase + r.lay.d_planes), s);
         if (r.m >= 2) {
-          const int v = default_variant(r.g.code.planes, r.m);
+          const int v = default_variant(r.g, true, r.max_dist);
           const uint64_t TI = 64ull * variant_of(v).R;
           run_allpairs(r.g, reinterpret_cast<uint32_t *>(dbase + r.lay.d_planes), r.out_bytes,
                        r.max_dist, true, v, 0, (r.m + TI - 1) / TI, dbase + r.lay.d_out, 0, s);

@@ -88,12 +88,17 @@ def test_invalid_arguments_are_rejected_before_device_use():
 def test_long_row_sparse_code_layout(monkeypatch):
     from hammingdist_amd import capi
 
-    # Rows of 2^28 sites or more take the class code: planes only, no counts
-    # (bytes_per_seq 2^27 = 2^28 sites = 2^23 words).
-    n_pad, W, P, total = capi.planes_layout(2, 1 << 27, capi.HD_SEM_SPARSE, 0xFFFF)
-    assert (n_pad, W, P) == (512, 1 << 23, 4) and total == 512 * (1 << 23) * 4
-    n_pad, W, P, total = capi.planes_layout(2, (1 << 27) - 16, capi.HD_SEM_SPARSE, 0xFFFF)
+    # Rows of 2^23 sites or more take the class code: planes only, no counts
+    # (bytes_per_seq 2^22 = 2^23 sites = 2^18 words); so does the dense
+    # semantics' nibble code.
+    n_pad, W, P, total = capi.planes_layout(2, 1 << 22, capi.HD_SEM_SPARSE, 0xFFFF)
+    assert (n_pad, W, P) == (512, 1 << 18, 4) and total == 512 * (1 << 18) * 4
+    n_pad, W, P, total = capi.planes_layout(2, (1 << 22) - 16, capi.HD_SEM_SPARSE, 0xFFFF)
     assert total > 512 * W * 4  # one word shorter: the gap code and its counts
+    acgt_gap = (1 << 1) | (1 << 2) | (1 << 4) | (1 << 8) | (1 << 15)
+    assert capi.planes_layout(2, 1 << 22, capi.HD_SEM_DENSE, acgt_gap)[3] == 512 * (1 << 18) * 4
+    assert capi.planes_layout(2, (1 << 22) - 16, capi.HD_SEM_DENSE, acgt_gap)[3] > \
+        512 * ((1 << 18) - 1) * 4
     monkeypatch.setenv("HD_SPARSE_CLASS_CODE", "1")
     n_pad, W, P, total = capi.planes_layout(1000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF)
     assert (n_pad, W, P) == (1024, 313, 4) and total == 1024 * 313 * 4
