@@ -201,9 +201,18 @@ public:
   void write_raw(const char *data, std::size_t len);
 
 private:
+  // Maps [offset_, offset_ + len) of the file for writing after reserving it
+  // with fallocate; null when the file cannot be written that way (not a
+  // regular file, no read access, no fallocate), and then pwrite is used.
+  char *map_range(std::uint64_t len, char **base, std::uint64_t *base_len);
+
   std::string filename_;
   int fd_ = -1;
+  bool mappable_ = false;
+  std::uint64_t map_bytes_ = 1ull << 30; // writes this large go through a mapping
   std::uint64_t offset_ = 0;
+  std::unique_ptr<char[]> buf_; // one-digit text of the last band, reused
+  std::uint64_t buf_cap_ = 0;
 };
 
 // Writes whole rows [row_begin, row_end) of a triangle whose first cell in

@@ -154,6 +154,27 @@ PYBIND11_MODULE(_hammingdist, m) {
       py::arg("filename"), py::arg("remove_duplicates") = false, py::arg("n") = 0,
       "(sequences as bytes, sequence indices) as read_fasta parses the file");
   m.def(
+      "_write_row_bands",
+      [](const std::string &filename, py::array cells, std::vector<std::uint64_t> bounds,
+         std::uint32_t max_value) {
+        // One RowWriter fed band by band, as the streaming triangle is.
+        const py::buffer_info info = cells.request();
+        if (info.ndim != 1 || (info.itemsize != 1 && info.itemsize != 2))
+          throw std::runtime_error("cells must be a flat uint8 or uint16 array");
+        RowWriter w(filename, false);
+        for (std::size_t b = 0; b + 1 < bounds.size(); ++b) {
+          const std::uint64_t r0 = bounds[b], c0 = r0 < 1 ? 0 : r0 * (r0 - 1) / 2;
+          if (info.itemsize == 1)
+            w.write(static_cast<const std::uint8_t *>(info.ptr) + c0, r0, bounds[b + 1],
+                    max_value);
+          else
+            w.write(static_cast<const std::uint16_t *>(info.ptr) + c0, r0, bounds[b + 1],
+                    max_value);
+        }
+      },
+      py::arg("filename"), py::arg("cells"), py::arg("bounds"), py::arg("max_value"),
+      "Writes the triangle's rows [bounds[b], bounds[b+1]) band after band");
+  m.def(
       "_pack",
       [](const std::vector<std::string> &data, bool include_x) {
         const std::uint64_t L = data.empty() ? 0 : data[0].size();

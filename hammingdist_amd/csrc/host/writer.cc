@@ -2,7 +2,10 @@
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
 #include <unistd.h>
 #include <cstring>
 #include <fstream>
@@ -98,13 +101,53 @@ void pwrite_all(int fd, const char *data, std::size_t len, std::uint64_t off) {
 
 RowWriter::RowWriter(const std::string &filename, bool append) : filename_(filename) {
   // No O_APPEND even when appending: on Linux pwrite on an O_APPEND
-  // descriptor ignores its offset, so the end is read once instead.
-  fd_ = ::open(filename.c_str(), O_WRONLY | O_CREAT | (append ? 0 : O_TRUNC), 0644);
+  // descriptor ignores its offset, so the end is read once instead.  Read
+  // access as well where the file allows it, for the mapped path.
+  const int flags = O_CREAT | (append ? 0 : O_TRUNC);
+  fd_ = ::open(filename.c_str(), O_RDWR | flags, 0644);
+  if (fd_ < 0) fd_ = ::open(filename.c_str(), O_WRONLY | flags, 0644);
+  else {
+    struct stat st;
+    mappable_ = ::fstat(fd_, &st) == 0 && S_ISREG(st.st_mode);
+  }
   if (fd_ < 0) throw std::runtime_error("Error: Failed to open file '" + filename + "'");
+  // HD_WRITER_MAP_BYTES overrides the mapping threshold (it lets the mapped
+  // path be exercised on small files).
+  if (const char *m = std::getenv("HD_WRITER_MAP_BYTES")) map_bytes_ = std::strtoull(m, nullptr, 10);
   if (append) {
     const off_t end = ::lseek(fd_, 0, SEEK_END);
     offset_ = end > 0 ? static_cast<std::uint64_t>(end) : 0;
   }
+}
+
+// Buffered writes into one file serialise on its inode lock; one large write
+// through a shared mapping faults pages in from every thread at once (2.5 GB
+// of text on ext4: 0.17 s against 0.38 s for 16 threads of pwrite).  Written
+// band after band through mappings the same text took 1.1 s on ext4 (0.24 s
+// with pwrite), so only writes of map_bytes_ (1 GiB) or more take this path.
+// fallocate first reserves the blocks, so the stores cannot hit a full disk;
+// a file system without fallocate keeps the pwrite path.
+char *RowWriter::map_range(std::uint64_t len, char **base, std::uint64_t *base_len) {
+  if (!mappable_ || len == 0) return nullptr;
+  const int fa = ::posix_fallocate(fd_, static_cast<off_t>(offset_), static_cast<off_t>(len));
+  if (fa != 0) {
+    if (fa == ENOSPC || fa == EFBIG || fa == EIO)
+      throw std::runtime_error(std::string("write to lower-triangular file failed: ") +
+                               std::strerror(fa));
+    mappable_ = false; // e.g. EOPNOTSUPP: posix_fallocate's emulation is not used
+    return nullptr;
+  }
+  const std::uint64_t page = static_cast<std::uint64_t>(::sysconf(_SC_PAGESIZE));
+  const std::uint64_t start = offset_ / page * page;
+  *base_len = offset_ + len - start;
+  void *m = ::mmap(nullptr, *base_len, PROT_READ | PROT_WRITE, MAP_SHARED, fd_,
+                   static_cast<off_t>(start));
+  if (m == MAP_FAILED) {
+    mappable_ = false;
+    return nullptr;
+  }
+  *base = static_cast<char *>(m);
+  return *base + (offset_ - start);
 }
 
 RowWriter::~RowWriter() {
@@ -139,6 +182,39 @@ void RowWriter::write(const T *cells, std::uint64_t row_begin, std::uint64_t row
   bounds.push_back(row_end);
   const std::size_t nb = bounds.size() - 1;
   const bool one_digit = max_value < 10;
+  if (one_digit) {
+    // Every cell is two bytes ("d," or "d\n"), so each chunk's place in the
+    // text is known in advance: chunks are formatted straight into the mapped
+    // file (one large write) or into the writer's reused buffer, which is
+    // then written by as many threads.
+    const std::uint64_t bytes = 2 * total;
+    char *base = nullptr;
+    std::uint64_t base_len = 0;
+    char *dst = bytes >= map_bytes_ ? map_range(bytes, &base, &base_len) : nullptr;
+    const bool mapped = dst != nullptr;
+    if (!mapped) {
+      if (buf_cap_ < bytes) {
+        buf_.reset(new char[bytes]); // no zero fill: every byte is formatted
+        buf_cap_ = bytes;
+      }
+      dst = buf_.get();
+    }
+    parallel_for(nb, [&](std::uint64_t b, std::uint64_t e) {
+      for (std::uint64_t k = b; k < e; ++k) {
+        const std::uint64_t r0 = bounds[k], r1 = bounds[k + 1];
+        if (r1 <= r0) continue;
+        const std::uint64_t before = r0 * (r0 - 1) / 2 - c0;
+        char *out = dst + 2 * before;
+        format_rows_1digit(cells + before, r0, r1, out);
+        if (!mapped)
+          pwrite_all(fd_, out, 2 * (r1 * (r1 - 1) / 2 - r0 * (r0 - 1) / 2),
+                     offset_ + 2 * before);
+      }
+    });
+    if (mapped) ::munmap(base, base_len);
+    offset_ += bytes;
+    return;
+  }
   std::vector<std::string> text(nb);
   parallel_for(nb, [&](std::uint64_t b, std::uint64_t e) {
     for (std::uint64_t k = b; k < e; ++k) {
@@ -146,21 +222,26 @@ void RowWriter::write(const T *cells, std::uint64_t row_begin, std::uint64_t row
       if (r1 <= r0) continue;
       const std::uint64_t cells_k = r1 * (r1 - 1) / 2 - r0 * (r0 - 1) / 2;
       const T *src = cells + (r0 * (r0 - 1) / 2 - c0);
-      if (one_digit) {
-        text[k].resize(2 * cells_k);
-        text[k].resize(format_rows_1digit(src, r0, r1, text[k].data()));
-      } else {
-        text[k].resize(cells_k * (kMaxDigits + 1) + 1);
-        text[k].resize(format_rows(src, r0, r1, text[k].data()));
-      }
+      text[k].resize(cells_k * (kMaxDigits + 1) + 1);
+      text[k].resize(format_rows(src, r0, r1, text[k].data()));
     }
   });
   std::vector<std::uint64_t> off(nb + 1, 0);
   for (std::size_t k = 0; k < nb; ++k) off[k + 1] = off[k] + text[k].size();
-  parallel_for(nb, [&](std::uint64_t b, std::uint64_t e) {
-    for (std::uint64_t k = b; k < e; ++k)
-      pwrite_all(fd_, text[k].data(), text[k].size(), offset_ + off[k]);
-  });
+  char *base = nullptr;
+  std::uint64_t base_len = 0;
+  if (char *dst = off[nb] >= map_bytes_ ? map_range(off[nb], &base, &base_len) : nullptr) {
+    parallel_for(nb, [&](std::uint64_t b, std::uint64_t e) {
+      for (std::uint64_t k = b; k < e; ++k)
+        std::memcpy(dst + off[k], text[k].data(), text[k].size());
+    });
+    ::munmap(base, base_len);
+  } else {
+    parallel_for(nb, [&](std::uint64_t b, std::uint64_t e) {
+      for (std::uint64_t k = b; k < e; ++k)
+        pwrite_all(fd_, text[k].data(), text[k].size(), offset_ + off[k]);
+    });
+  }
   offset_ += off[nb];
 }
 

@@ -249,3 +249,30 @@ def test_generator_is_deterministic():
     txt = {bytes(r) for r in a}
     assert len(txt) < 50  # duplicates present
     assert set(np.unique(a).tobytes()) <= set(b"ACGTN-")
+
+
+@pytest.mark.parametrize("map_bytes", [None, "1"])
+@pytest.mark.parametrize("top", [3, 10, 70000])
+def test_row_writer_bands_match_oracle(tmp_path, top, map_bytes, rng, monkeypatch):
+    # The streaming writer's use: one file written band after band, at file
+    # offsets that are not page aligned.  Bands are written with pwrite, or
+    # (HD_WRITER_MAP_BYTES=1: the path of writes of 1 GiB and more) through
+    # a mapping of the file: one-digit text formatted straight into it, other
+    # text copied into it.
+    from hammingdist_amd import _native
+
+    if map_bytes:
+        monkeypatch.setenv("HD_WRITER_MAP_BYTES", map_bytes)
+
+    n = 900
+    dt = np.uint8 if top < 256 else np.uint16
+    cells = rng.integers(0, top, size=n * (n - 1) // 2).astype(dt)
+    want = str(tmp_path / "want.txt")
+    O.lib().oracle_write_lower_triangular(want.encode(), cells.ctypes.data, n, cells.itemsize)
+    bounds = sorted({0, n} | {int(x) for x in rng.integers(1, n, size=7)})
+    out = tmp_path / "out.txt"
+    out.write_bytes(b"stale" * 400000)  # longer than the result: truncated on open
+    _native._write_row_bands(str(out), cells, bounds, int(cells.max()))
+    assert out.read_bytes() == open(want, "rb").read()
+    # not a regular file: the pwrite path
+    _native._write_row_bands("/dev/null", cells, bounds, int(cells.max()))
