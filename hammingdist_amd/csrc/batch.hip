@@ -66,19 +66,19 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Encode tiles of every job (kernels.h, encode_tile): workgroup b of job j
+// takes its tile (row group, word block) from b - block_begin.
 __global__ __launch_bounds__(256) void k_batch_encode(const EncodeJobDev *__restrict__ jobs,
                                                       uint32_t n_jobs) {
   __shared__ uint8_t s_tbl[16];
+  __shared__ __attribute__((aligned(16))) uint8_t s_rows[kEncodeLdsBytes];
   const uint32_t j = find_job(jobs, n_jobs, blockIdx.x, &EncodeJobDev::block_begin);
   const EncodeJobDev &J = jobs[j];
   if (threadIdx.x < 16) s_tbl[threadIdx.x] = J.tbl.v[threadIdx.x];
-  __syncthreads();
   const uint64_t local = blockIdx.x - J.block_begin;
-  const uint32_t row_blocks = J.n_pad / 256;
-  const uint32_t i = (uint32_t)(local % row_blocks) * 256 + threadIdx.x;
-  const uint32_t run = (uint32_t)(local / row_blocks);
-  encode_run(J.packed, J.n, J.bpr, J.W, (int)J.P, s_tbl, J.planes, i, run * kEncodeRun,
-             (int)J.mode);
+  const uint32_t groups = J.n_pad / kWave;
+  encode_tile(J.packed, J.n, J.bpr, J.W, (int)J.P, s_tbl, (int)J.mode, J.planes,
+              (uint32_t)(local % groups), (uint32_t)(local / groups) * kEncodeWords, s_rows);
 }
 
 // Correction counts of every job whose code has them: one workgroup per row
@@ -343,10 +343,10 @@ void prepare(hd_batch *b, int variant, int early_exit) {
     e.W = J.g.W;
     e.P = (uint32_t)J.g.code.planes;
     e.op = (uint32_t)J.g.code.op;
-    e.runs = (J.g.W + kEncodeRun - 1) / kEncodeRun;
+    e.runs = (J.g.W + kEncodeWords - 1) / kEncodeWords; // word blocks
     std::memcpy(e.tbl.v, J.g.code.table, 16);
     e.mode = (uint32_t)encode_mode(J.g.code);
-    blocks += (uint64_t)(e.n_pad / 256) * e.runs;
+    blocks += (uint64_t)(e.n_pad / kWave) * e.runs;
     if (has_corr(J.g.code.op)) count_blocks += e.n_pad / 64;
     enc.push_back(e);
     if (J.n < 2) continue;

@@ -279,21 +279,17 @@ static void check_max_dist(int out_bytes, uint32_t max_dist) {
 // ---------------------------------------------------------------------------
 // encode: packed nibble rows -> bit-sliced planes
 // ---------------------------------------------------------------------------
-// One thread per (row, run of 8 words): it reads 128 contiguous bytes of its
-// row and writes 8 x P plane words; the 64 rows of a wavefront form one row
-// group, so each store is 256 contiguous bytes.  Rows >= n and sites past the row end encode as '-' (all
-// planes set), which never counts against anything.
+// Workgroup b encodes tile (row group b % groups, words (b / groups) *
+// kEncodeWords ...) through LDS (kernels.h, encode_tile).
 __global__ __launch_bounds__(256) void k_encode(const uint8_t *__restrict__ packed,
-                                                uint32_t n, uint64_t bpr,
-                                                uint32_t n_pad, uint32_t W,
-                                                int P, Table16 tbl, int mode,
+                                                uint32_t n, uint64_t bpr, uint32_t groups,
+                                                uint32_t W, int P, Table16 tbl, int mode,
                                                 uint32_t *__restrict__ planes) {
   __shared__ uint8_t s_tbl[16];
+  __shared__ __attribute__((aligned(16))) uint8_t s_rows[kEncodeLdsBytes];
   if (threadIdx.x < 16) s_tbl[threadIdx.x] = tbl.v[threadIdx.x];
-  __syncthreads();
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_pad) return;
-  encode_run(packed, n, bpr, W, P, s_tbl, planes, i, blockIdx.y * kEncodeRun, mode);
+  const uint32_t g = blockIdx.x % groups, kt = blockIdx.x / groups;
+  encode_tile(packed, n, bpr, W, P, s_tbl, mode, planes, g, kt * kEncodeWords, s_rows);
 }
 
 // Correction counts of the kDense3 and kGap codes, one workgroup per row
@@ -319,12 +315,12 @@ void launch_encode(const Geometry &g, const uint8_t *d_packed, uint32_t *d_plane
                    hipStream_t s) {
   Table16 t;
   std::memcpy(t.v, g.code.table, 16);
-  const uint32_t runs = (g.W + kEncodeRun - 1) / kEncodeRun;
-  HD_REQUIRE(runs <= 65535, "sequences too long for the encode grid");
-  dim3 grid((uint32_t)((g.n_pad + 255) / 256), runs);
-  hipLaunchKernelGGL(k_encode, grid, dim3(256), 0, s, d_packed, (uint32_t)g.n,
-                     g.bpr, (uint32_t)g.n_pad, g.W, g.code.planes, t, encode_mode(g.code),
-                     d_planes);
+  const uint64_t groups = g.n_pad / kWave;
+  const uint64_t blocks = groups * ((g.W + kEncodeWords - 1) / kEncodeWords);
+  HD_REQUIRE(blocks <= 0x7fffffffull, "dataset too large for the encode grid");
+  hipLaunchKernelGGL(k_encode, dim3((uint32_t)blocks), dim3(256), 0, s, d_packed,
+                     (uint32_t)g.n, g.bpr, (uint32_t)groups, g.W, g.code.planes, t,
+                     encode_mode(g.code), d_planes);
   HD_HIP(hipGetLastError());
   if (has_corr(g.code.op)) {
     hipLaunchKernelGGL(k_corr_counts, dim3((uint32_t)(g.n_pad / 64)), dim3(256), 0, s,

@@ -150,7 +150,7 @@ PlaneCode plane_code(int semantics, uint32_t nibble_mask, uint32_t words = 0);
 bool is_identity(const PlaneCode &c);
 // True for the dense 3-op code.
 inline bool is_dense3(const PlaneCode &c) { return c.op == 3; }
-// encode_run mode (kernels.h EncodeMode) for a code.
+// encode_tile mode (kernels.h EncodeMode) for a code.
 int encode_mode(const PlaneCode &c);
 
 inline uint64_t tri_cells(uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; }

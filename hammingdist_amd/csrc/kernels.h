@@ -413,11 +413,17 @@ __global__ __launch_bounds__(kWave *NW, 4) void k_allpairs_gap(
 // ---------------------------------------------------------------------------
 // encode: packed nibble rows -> bit-sliced planes
 // ---------------------------------------------------------------------------
-// Row i, words k0 .. k0+7: reads 128 contiguous bytes of the row and writes
-// 8 x P plane words; the 64 rows of a wavefront form one row group, so each
-// store is 256 contiguous bytes.  Rows >= n and sites past the row end
-// encode as '-' (table entry 15, all planes set), which never counts.
-constexpr int kEncodeRun = 8;
+// A 256-thread workgroup encodes one tile: row group g (64 rows) by words
+// k0 .. k0 + kEncodeWords - 1 (256 bytes of each row).  The rows' bytes are
+// read coalesced (16 threads per row segment) into LDS; then lane r of each
+// wavefront takes row r, wavefront w its words w, w + 4, ..., and each
+// store of one word of one plane for the 64 rows is 256 contiguous bytes.
+// Rows >= n and sites past the row end encode as '-' (nibble 15).
+constexpr int kEncodeWords = 16;
+// LDS row stride in bytes: 16 bytes of padding make lanes reading the same
+// word of consecutive rows fall into different banks.
+constexpr int kEncodeStride = kEncodeWords * 16 + 16;
+constexpr int kEncodeLdsBytes = kWave * kEncodeStride;
 
 // Bit p of each of the 8 nibbles of v, gathered into bits 0..7 (nibble k ->
 // bit k): mask the bits, then halve the spacing three times.
@@ -428,89 +434,80 @@ __device__ __forceinline__ uint32_t nibble_bits(uint32_t v, int p) {
   return (x | (x >> 12)) & 0xFFu;
 }
 
-// How encode_run builds the planes: through the 16-entry table, as the
-// nibble itself (4 planes), or as the kDense3 planes derived from the nibble
-// bits (valid for nibbles 0, 1, 2, 4, 8 and 15 only).
+// How the planes are built: through the 16-entry table, as the nibble itself
+// (4 planes), or as the kDense3 planes derived from the nibble bits (valid
+// for nibbles 0, 1, 2, 4, 8 and 15 only).
 enum EncodeMode { kEncTable = 0, kEncIdentity = 1, kEncDense3 = 2 };
 
-__device__ __forceinline__ void encode_run(const uint8_t *__restrict__ packed, uint32_t n,
-                                           uint64_t bpr, uint32_t W, int P,
-                                           const uint8_t *s_tbl,
-                                           uint32_t *__restrict__ planes, uint32_t i,
-                                           uint32_t k0, int mode) {
-  const uint8_t *row = packed + (uint64_t)i * bpr;
+// Planes of one 32-site word (16 bytes, sites 2b and 2b + 1 in byte b) into
+// w[0 .. P-1].
+__device__ __forceinline__ void encode_word(const uint32_t (&bytes)[4], int P,
+                                            const uint8_t *s_tbl, int mode, uint32_t (&w)[6]) {
   if (mode != kEncTable) {
     // Plane p of the nibble code is bit p of every nibble, with no table
     // lookup; kDense3 planes are a few bitwise ops on those.
-    for (int kk = 0; kk < kEncodeRun; ++kk) {
-      const uint32_t k = k0 + kk;
-      if (k >= W) break;
-      const uint64_t off = (uint64_t)k * 16;
-      uint32_t bytes[4];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const uint64_t o = off + 8 * h;
-        if (i < n && o + 8 <= bpr) {
-          const uint2 u = *reinterpret_cast<const uint2 *>(row + o);
-          bytes[2 * h] = u.x;
-          bytes[2 * h + 1] = u.y;
-        } else {
-          bytes[2 * h] = 0xFFFFFFFFu;
-          bytes[2 * h + 1] = 0xFFFFFFFFu;
-        }
-      }
-      uint32_t nb[4];
-#pragma unroll
-      for (int p = 0; p < 4; ++p)
-        nb[p] = nibble_bits(bytes[0], p) | (nibble_bits(bytes[1], p) << 8) |
-                (nibble_bits(bytes[2], p) << 16) | (nibble_bits(bytes[3], p) << 24);
-      if (mode == kEncDense3) {
-        // Among 0, A=1, C=2, G=4, T=8 and '-'=15 only '-' has bits 0 and 1.
-        const uint32_t g = nb[0] & nb[1];
-        const uint32_t c1 = nb[0] | nb[2];                       // A G -
-        const uint32_t c2 = nb[0] ^ nb[1];                       // A C
-        const uint32_t c3 = c2 | ((nb[2] | nb[3]) & ~nb[0]);     // A C G T
-        nb[0] = g;
-        nb[1] = c1;
-        nb[2] = c2;
-        nb[3] = c3;
-      }
-      uint32_t *dst = planes + (((uint64_t)(i / 64) * W + k) * 4) * 64 + (i % 64);
-#pragma unroll
-      for (int p = 0; p < 4; ++p) dst[p * 64] = nb[p];
+    for (int p = 0; p < 4; ++p)
+      w[p] = nibble_bits(bytes[0], p) | (nibble_bits(bytes[1], p) << 8) |
+             (nibble_bits(bytes[2], p) << 16) | (nibble_bits(bytes[3], p) << 24);
+    if (mode == kEncDense3) {
+      // Among 0, A=1, C=2, G=4, T=8 and '-'=15 only '-' has bits 0 and 1.
+      const uint32_t n0 = w[0], n1 = w[1], n2 = w[2], n3 = w[3];
+      const uint32_t c2 = n0 ^ n1;                   // A C
+      w[0] = n0 & n1;                                // g: '-'
+      w[1] = n0 | n2;                                // A G -
+      w[2] = c2;
+      w[3] = c2 | ((n2 | n3) & ~n0);                 // A C G T
     }
     return;
   }
-  for (int kk = 0; kk < kEncodeRun; ++kk) {
-    const uint32_t k = k0 + kk;
-    if (k >= W) break;
-    uint32_t w[6] = {0, 0, 0, 0, 0, 0};
-    const uint64_t off = (uint64_t)k * 16;
-    uint32_t bytes[4];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint64_t o = off + 8 * h;
-      if (i < n && o + 8 <= bpr) {
-        const uint2 u = *reinterpret_cast<const uint2 *>(row + o);
-        bytes[2 * h] = u.x;
-        bytes[2 * h + 1] = u.y;
-      } else {
-        bytes[2 * h] = 0xFFFFFFFFu;
-        bytes[2 * h + 1] = 0xFFFFFFFFu;
-      }
-    }
+  for (int p = 0; p < 6; ++p) w[p] = 0;
 #pragma unroll
-    for (int s = 0; s < 32; ++s) {
-      const uint32_t nib = (bytes[s >> 3] >> (4 * (s & 7))) & 15u;
-      const uint32_t m = s_tbl[nib];
-#pragma unroll
-      for (int p = 0; p < 6; ++p)
-        if (p < P) w[p] |= ((m >> p) & 1u) << s;
-    }
-    uint32_t *dst = planes + (((uint64_t)(i / 64) * W + k) * P) * 64 + (i % 64);
+  for (int st = 0; st < 32; ++st) {
+    const uint32_t m = s_tbl[(bytes[st >> 3] >> (4 * (st & 7))) & 15u];
 #pragma unroll
     for (int p = 0; p < 6; ++p)
-      if (p < P) dst[p * 64] = w[p];
+      if (p < P) w[p] |= ((m >> p) & 1u) << st;
+  }
+}
+
+// The tile (g, k0) of a dataset of n rows of bpr bytes, W words, P stored
+// planes; called by all 256 threads of a workgroup with `lds` holding
+// kEncodeLdsBytes bytes.
+__device__ __forceinline__ void encode_tile(const uint8_t *__restrict__ packed, uint32_t n,
+                                            uint64_t bpr, uint32_t W, int P,
+                                            const uint8_t *s_tbl, int mode,
+                                            uint32_t *__restrict__ planes, uint32_t g,
+                                            uint32_t k0, uint8_t *lds) {
+  const uint32_t tid = threadIdx.x;
+#pragma unroll
+  for (int it = 0; it < kWave * kEncodeWords / 256; ++it) {
+    const uint32_t c = tid + 256u * it, r = c / kEncodeWords, part = c % kEncodeWords;
+    const uint32_t row = g * kWave + r, k = k0 + part;
+    uint2 lo = make_uint2(~0u, ~0u), hi = lo;
+    if (row < n && k < W) {
+      const uint8_t *src = packed + (uint64_t)row * bpr + (uint64_t)k * 16;
+      const uint64_t off = (uint64_t)k * 16;
+      if (off + 8 <= bpr) lo = *reinterpret_cast<const uint2 *>(src);
+      if (off + 16 <= bpr) hi = *reinterpret_cast<const uint2 *>(src + 8);
+    }
+    *reinterpret_cast<uint4 *>(lds + r * kEncodeStride + part * 16) =
+        make_uint4(lo.x, lo.y, hi.x, hi.y);
+  }
+  __syncthreads();
+  const uint32_t r = tid % kWave, w0 = tid / kWave;
+  for (uint32_t part = w0; part < (uint32_t)kEncodeWords; part += 256 / kWave) {
+    const uint32_t k = k0 + part;
+    if (k >= W) break;
+    const uint4 v = *reinterpret_cast<const uint4 *>(lds + r * kEncodeStride + part * 16);
+    const uint32_t bytes[4] = {v.x, v.y, v.z, v.w};
+    uint32_t w[6];
+    encode_word(bytes, P, s_tbl, mode, w);
+    uint32_t *dst = planes + (((uint64_t)g * W + k) * P) * kWave + r;
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+      if (p < P) dst[p * kWave] = w[p];
   }
 }
 
