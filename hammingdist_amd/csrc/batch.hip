@@ -39,7 +39,7 @@ struct EncodeJobDev {
   uint32_t *corr;       // correction counts (codes with has_corr), else null
   uint64_t bpr;
   uint64_t block_begin; // first encode block of this job
-  uint64_t count_begin; // first count block (row group) of this job
+  uint64_t count_begin; // first totals block (256 rows) of this job
   uint32_t n, n_pad, W, P, runs, mode;
   uint32_t op; // plane code (kernels.h), for the correction counts
   Table16 tbl;
@@ -77,18 +77,20 @@ __global__ __launch_bounds__(256) void k_batch_encode(const EncodeJobDev *__rest
   if (threadIdx.x < 16) s_tbl[threadIdx.x] = J.tbl.v[threadIdx.x];
   const uint64_t local = blockIdx.x - J.block_begin;
   const uint32_t groups = J.n_pad / kWave;
-  encode_tile(J.packed, J.n, J.bpr, J.W, (int)J.P, s_tbl, (int)J.mode, J.planes,
-              (uint32_t)(local % groups), (uint32_t)(local / groups) * kEncodeWords, s_rows);
+  encode_tile(J.packed, J.n, J.bpr, J.W, (int)J.P, (int)J.op, s_tbl, (int)J.mode, J.planes,
+              J.corr, J.n_pad, (uint32_t)(local % groups),
+              (uint32_t)(local / groups) * kEncodeWords, s_rows);
 }
 
-// Correction counts of every job whose code has them: one workgroup per row
-// group.
-__global__ __launch_bounds__(256) void k_batch_counts(const EncodeJobDev *__restrict__ jobs,
+// Whole-row correction counts of every job whose code has them, from the
+// chunk counts its encode tiles wrote: one thread per row, 256 rows per
+// workgroup.
+__global__ __launch_bounds__(256) void k_batch_totals(const EncodeJobDev *__restrict__ jobs,
                                                       uint32_t n_jobs) {
   const uint32_t j = find_job(jobs, n_jobs, blockIdx.x, &EncodeJobDev::count_begin);
   const EncodeJobDev &J = jobs[j];
-  corr_counts_group(J.planes, J.W, (uint32_t)(blockIdx.x - J.count_begin), J.n_pad,
-                    (int)J.op, J.corr);
+  const uint32_t row = (uint32_t)(blockIdx.x - J.count_begin) * 256 + threadIdx.x;
+  corr_totals_row(J.corr, J.W, J.n_pad, (int)J.op, row);
 }
 
 __global__ void k_reset_counter(unsigned long long *counter) { *counter = 0; }
@@ -278,7 +280,7 @@ void enqueue_batch(hd_batch *b, hipStream_t s) {
   }
   if (b->count_blocks) {
     HD_REQUIRE(b->count_blocks < (1ull << 31), "batch too large for one count launch");
-    hipLaunchKernelGGL(k_batch_counts, dim3((uint32_t)b->count_blocks), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_batch_totals, dim3((uint32_t)b->count_blocks), dim3(256), 0, s,
                        b->d_enc.get<EncodeJobDev>(), (uint32_t)b->jobs.size());
     HD_HIP(hipGetLastError());
   }
@@ -347,7 +349,7 @@ void prepare(hd_batch *b, int variant, int early_exit) {
     std::memcpy(e.tbl.v, J.g.code.table, 16);
     e.mode = (uint32_t)encode_mode(J.g.code);
     blocks += (uint64_t)(e.n_pad / kWave) * e.runs;
-    if (has_corr(J.g.code.op)) count_blocks += e.n_pad / 64;
+    if (has_corr(J.g.code.op)) count_blocks += e.n_pad / 256;
     enc.push_back(e);
     if (J.n < 2) continue;
     BatchJobDev d{};

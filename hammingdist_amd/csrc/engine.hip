@@ -283,21 +283,23 @@ static void check_max_dist(int out_bytes, uint32_t max_dist) {
 // kEncodeWords ...) through LDS (kernels.h, encode_tile).
 __global__ __launch_bounds__(256) void k_encode(const uint8_t *__restrict__ packed,
                                                 uint32_t n, uint64_t bpr, uint32_t groups,
-                                                uint32_t W, int P, Table16 tbl, int mode,
-                                                uint32_t *__restrict__ planes) {
+                                                uint32_t W, int P, int op, Table16 tbl,
+                                                int mode, uint32_t *__restrict__ planes,
+                                                uint32_t *__restrict__ corr) {
   __shared__ uint8_t s_tbl[16];
   __shared__ __attribute__((aligned(16))) uint8_t s_rows[kEncodeLdsBytes];
   if (threadIdx.x < 16) s_tbl[threadIdx.x] = tbl.v[threadIdx.x];
   const uint32_t g = blockIdx.x % groups, kt = blockIdx.x / groups;
-  encode_tile(packed, n, bpr, W, P, s_tbl, mode, planes, g, kt * kEncodeWords, s_rows);
+  encode_tile(packed, n, bpr, W, P, op, s_tbl, mode, planes, corr, groups * kWave, g,
+              kt * kEncodeWords, s_rows);
 }
 
-// Correction counts of the kDense3 and kGap codes, one workgroup per row
-// group.
-__global__ __launch_bounds__(256) void k_corr_counts(const uint32_t *__restrict__ planes,
-                                                     uint32_t W, uint32_t n_pad, int P,
-                                                     uint32_t *__restrict__ corr) {
-  corr_counts_group(planes, W, blockIdx.x, n_pad, P, corr);
+// Whole-row correction counts of the kDense3 and kGap codes from the chunk
+// counts encode wrote, one thread per row.
+__global__ __launch_bounds__(256) void k_corr_totals(uint32_t *__restrict__ corr, uint32_t W,
+                                                     uint32_t n_pad, int op) {
+  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row < n_pad) corr_totals_row(corr, W, n_pad, op, row);
 }
 
 int encode_mode(const PlaneCode &c) {
@@ -318,14 +320,14 @@ void launch_encode(const Geometry &g, const uint8_t *d_packed, uint32_t *d_plane
   const uint64_t groups = g.n_pad / kWave;
   const uint64_t blocks = groups * ((g.W + kEncodeWords - 1) / kEncodeWords);
   HD_REQUIRE(blocks <= 0x7fffffffull, "dataset too large for the encode grid");
+  uint32_t *corr = has_corr(g.code.op) ? d_planes + g.plane_words() : nullptr;
   hipLaunchKernelGGL(k_encode, dim3((uint32_t)blocks), dim3(256), 0, s, d_packed,
-                     (uint32_t)g.n, g.bpr, (uint32_t)groups, g.W, g.code.planes, t,
-                     encode_mode(g.code), d_planes);
+                     (uint32_t)g.n, g.bpr, (uint32_t)groups, g.W, g.code.planes, g.code.op,
+                     t, encode_mode(g.code), d_planes, corr);
   HD_HIP(hipGetLastError());
-  if (has_corr(g.code.op)) {
-    hipLaunchKernelGGL(k_corr_counts, dim3((uint32_t)(g.n_pad / 64)), dim3(256), 0, s,
-                       d_planes, g.W, (uint32_t)g.n_pad, g.code.op,
-                       d_planes + g.plane_words());
+  if (corr) {
+    hipLaunchKernelGGL(k_corr_totals, dim3((uint32_t)(g.n_pad / 256)), dim3(256), 0, s, corr,
+                       g.W, (uint32_t)g.n_pad, g.code.op);
     HD_HIP(hipGetLastError());
   }
 }

@@ -414,12 +414,16 @@ __global__ __launch_bounds__(kWave *NW, 4) void k_allpairs_gap(
 // encode: packed nibble rows -> bit-sliced planes
 // ---------------------------------------------------------------------------
 // A 256-thread workgroup encodes one tile: row group g (64 rows) by words
-// k0 .. k0 + kEncodeWords - 1 (256 bytes of each row).  The rows' bytes are
-// read coalesced (16 threads per row segment) into LDS; then lane r of each
-// wavefront takes row r, wavefront w its words w, w + 4, ..., and each
-// store of one word of one plane for the 64 rows is 256 contiguous bytes.
-// Rows >= n and sites past the row end encode as '-' (nibble 15).
-constexpr int kEncodeWords = 16;
+// k0 .. k0 + kEncodeWords - 1 (512 bytes of each row; one chunk of
+// kCheckWords words).  The rows' bytes are read coalesced (32 threads per
+// row segment) into LDS; then lane r of each wavefront takes row r,
+// wavefront w its words w, w + 4, ..., and each store of one word of one
+// plane for the 64 rows is 256 contiguous bytes.  For the codes with
+// correction counts the tile also writes its chunk's counts (the layout of
+// dense3_corr_words / gap_corr_words); the whole-row slots are summed
+// afterwards (corr_totals_row).  Rows >= n and sites past the row end
+// encode as '-' (nibble 15).
+constexpr int kEncodeWords = 32;
 // LDS row stride in bytes: 16 bytes of padding make lanes reading the same
 // word of consecutive rows fall into different banks.
 constexpr int kEncodeStride = kEncodeWords * 16 + 16;
@@ -473,13 +477,17 @@ __device__ __forceinline__ void encode_word(const uint32_t (&bytes)[4], int P,
 }
 
 // The tile (g, k0) of a dataset of n rows of bpr bytes, W words, P stored
-// planes; called by all 256 threads of a workgroup with `lds` holding
-// kEncodeLdsBytes bytes.
+// planes, plane code `op`; called by all 256 threads of a workgroup with
+// `lds` holding kEncodeLdsBytes bytes.  corr: the correction counts after
+// the planes when has_corr(op), else null.
 __device__ __forceinline__ void encode_tile(const uint8_t *__restrict__ packed, uint32_t n,
-                                            uint64_t bpr, uint32_t W, int P,
+                                            uint64_t bpr, uint32_t W, int P, int op,
                                             const uint8_t *s_tbl, int mode,
-                                            uint32_t *__restrict__ planes, uint32_t g,
-                                            uint32_t k0, uint8_t *lds) {
+                                            uint32_t *__restrict__ planes,
+                                            uint32_t *__restrict__ corr, uint32_t n_pad,
+                                            uint32_t g, uint32_t k0, uint8_t *lds) {
+  static_assert(kEncodeWords == (int)kCheckWords, "one encode tile per count chunk");
+  __shared__ uint32_t s_cnt[2][256 / kWave][kWave];
   const uint32_t tid = threadIdx.x;
 #pragma unroll
   for (int it = 0; it < kWave * kEncodeWords / 256; ++it) {
@@ -497,6 +505,11 @@ __device__ __forceinline__ void encode_tile(const uint8_t *__restrict__ packed, 
   }
   __syncthreads();
   const uint32_t r = tid % kWave, w0 = tid / kWave;
+  const bool d3 = op == kDense3;
+  const int gp = op == kGap2 ? 2 : 3; // '-' plane of the kGap codes
+  // kDense3: a = invalid sites, b = '-' sites; kGap*: a = '-' sites, b =
+  // bitmap of the words holding a '-' (bit k - k0).
+  uint32_t ca = 0, cb = 0;
   for (uint32_t part = w0; part < (uint32_t)kEncodeWords; part += 256 / kWave) {
     const uint32_t k = k0 + part;
     if (k >= W) break;
@@ -508,61 +521,47 @@ __device__ __forceinline__ void encode_tile(const uint8_t *__restrict__ packed, 
 #pragma unroll
     for (int p = 0; p < 6; ++p)
       if (p < P) dst[p * kWave] = w[p];
+    if (corr) {
+      if (d3) {
+        ca += __builtin_popcount(~(w[0] | w[1] | w[2] | w[3]));
+        cb += __builtin_popcount(w[0]);
+      } else {
+        const uint32_t gw = gp == 2 ? w[2] : w[3];
+        ca += __builtin_popcount(gw);
+        cb |= (gw != 0 ? 1u : 0u) << part;
+      }
+    }
+  }
+  if (!corr) return;
+  s_cnt[0][w0][r] = ca;
+  s_cnt[1][w0][r] = cb;
+  __syncthreads();
+  if (w0 == 0) {
+    uint32_t a = s_cnt[0][0][r], b = s_cnt[1][0][r];
+#pragma unroll
+    for (int q = 1; q < 256 / kWave; ++q) {
+      a += s_cnt[0][q][r];
+      b = d3 ? b + s_cnt[1][q][r] : (b | s_cnt[1][q][r]);
+    }
+    const uint64_t T = (W + kCheckWords - 1) / kCheckWords, t = k0 / kCheckWords;
+    const uint64_t row = (uint64_t)g * kWave + r;
+    corr[(2 + t) * n_pad + row] = a;
+    corr[(2 + T + t) * n_pad + row] = b;
   }
 }
 
-// Per-row correction counts of row group g (64 rows), layout
-// dense3_corr_words / gap_corr_words.  kDense3: Z = invalid sites (all four
-// planes clear) and G = '-' sites (plane 0).  kGap2 / kGap3: G = '-' sites
-// (plane B) in both whole-row slots, per chunk G and the bitmap of the
-// chunk's words that hold a '-'.  Per chunk of kCheckWords words and over
-// the whole row.  One 256-thread workgroup per group: wave w counts the
-// chunks t = w mod 4 for the 64 rows (lane = row, 256-byte loads), then
-// wave 0 adds the four partial row totals.
-__device__ __forceinline__ void corr_counts_group(const uint32_t *__restrict__ planes,
-                                                  uint32_t W, uint32_t g, uint32_t n_pad,
-                                                  int P, uint32_t *__restrict__ corr) {
-  __shared__ uint32_t s_z[4][kWave], s_g[4][kWave];
-  const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const uint32_t row = g * kWave + lane;
+// Whole-row slots of the correction counts of `row` from its chunk counts:
+// kDense3 Z and G, kGap* G twice.
+__device__ __forceinline__ void corr_totals_row(uint32_t *__restrict__ corr, uint32_t W,
+                                                uint32_t n_pad, int op, uint32_t row) {
   const uint32_t T = (W + kCheckWords - 1) / kCheckWords;
-  const bool d3 = P == kDense3;
-  const uint32_t PS = (uint32_t)stored_planes(P);
-  const uint32_t gp = P == kGap2 ? 2 : 3; // '-' plane of the kGap codes
-  const uint32_t *base = planes + (size_t)g * W * PS * kWave + lane;
-  uint32_t z = 0, gc = 0;
-  for (uint32_t t = wave; t < T; t += 4) {
-    uint32_t zt = 0, gt = 0;
-    const uint32_t k0 = t * kCheckWords, k_end = min(W, k0 + kCheckWords);
-    if (d3) {
-      for (uint32_t k = k0; k < k_end; ++k) {
-        const uint32_t *w = base + (size_t)k * 4 * kWave;
-        const uint32_t p0 = w[0], p1 = w[kWave], p2 = w[2 * kWave], p3 = w[3 * kWave];
-        zt += __builtin_popcount(~(p0 | p1 | p2 | p3));
-        gt += __builtin_popcount(p0);
-      }
-    } else {
-      // zt: bitmap of the words with a '-'
-      for (uint32_t k = k0; k < k_end; ++k) {
-        const uint32_t gw = base[((size_t)k * PS + gp) * kWave];
-        gt += __builtin_popcount(gw);
-        zt |= (gw != 0 ? 1u : 0u) << (k - k0);
-      }
-    }
-    corr[(2 + (uint64_t)t) * n_pad + row] = d3 ? zt : gt;
-    corr[(2 + (uint64_t)T + t) * n_pad + row] = d3 ? gt : zt;
-    z += zt;
-    gc += gt;
+  uint32_t a = 0, b = 0;
+  for (uint32_t t = 0; t < T; ++t) {
+    a += corr[(2 + (uint64_t)t) * n_pad + row];
+    b += corr[(2 + (uint64_t)T + t) * n_pad + row];
   }
-  s_z[wave][lane] = z;
-  s_g[wave][lane] = gc;
-  __syncthreads();
-  if (wave == 0) {
-    const uint32_t zs = s_z[0][lane] + s_z[1][lane] + s_z[2][lane] + s_z[3][lane];
-    const uint32_t gs = s_g[0][lane] + s_g[1][lane] + s_g[2][lane] + s_g[3][lane];
-    corr[row] = d3 ? zs : gs;
-    corr[n_pad + row] = gs;
-  }
+  corr[row] = a;
+  corr[n_pad + row] = op == kDense3 ? b : a;
 }
 
 // Number of tiles of the rows blocks [0, rb) for an M column blocks per row
