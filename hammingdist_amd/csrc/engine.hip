@@ -633,7 +633,7 @@ __global__ __launch_bounds__(256) void k_emit_write(const T *__restrict__ cells,
                                                     uint64_t n_cells, uint32_t thr, int vec,
                                                     const uint32_t *__restrict__ local_prefix,
                                                     const uint64_t *__restrict__ chunk_prefix,
-                                                    uint64_t capacity,
+                                                    uint64_t capacity, int vec_out,
                                                     uint32_t *__restrict__ trip) {
   const uint64_t x0 = (uint64_t)blockIdx.x * kEmitCells + threadIdx.x * kEmitPerThread;
   uint32_t v[16];
@@ -671,8 +671,23 @@ __global__ __launch_bounds__(256) void k_emit_write(const T *__restrict__ cells,
     }
   }
   __syncthreads();
-  const uint64_t words = 3 * total, first = 3 * base, limit = 3 * capacity;
-  for (uint64_t w = threadIdx.x; w < words && first + w < limit; w += 256)
+  const uint64_t first = 3 * base, limit = 3 * capacity;
+  const uint64_t words = first >= limit ? 0 : min(3 * total, limit - first);
+  if (!vec_out) {
+    for (uint64_t w = threadIdx.x; w < words; w += 256) trip[first + w] = s_trip[w];
+    return;
+  }
+  // 16-byte stores from the first 16-byte boundary of the output on (the
+  // output itself is 16-byte aligned); single words before and after.
+  const uint32_t head = (uint32_t)min<uint64_t>((4u - (uint32_t)(first & 3u)) & 3u, words);
+  const uint64_t quads = (words - head) / 4;
+  if (threadIdx.x < head) trip[first + threadIdx.x] = s_trip[threadIdx.x];
+  uint4 *dst = reinterpret_cast<uint4 *>(trip + first + head);
+  for (uint64_t q = threadIdx.x; q < quads; q += 256) {
+    const uint32_t *src = s_trip + head + 4 * q;
+    dst[q] = make_uint4(src[0], src[1], src[2], src[3]);
+  }
+  for (uint64_t w = head + 4 * quads + threadIdx.x; w < words; w += 256)
     trip[first + w] = s_trip[w];
 }
 
@@ -700,9 +715,10 @@ static uint64_t emit_t(const T *d_cells, uint64_t n_cells, uint32_t thr,
                         hipMemcpyDeviceToHost, s));
   HD_HIP(hipStreamSynchronize(s));
   if (total && capacity) {
+    const int vec_out = (reinterpret_cast<uintptr_t>(d_trip) % 16) == 0 ? 1 : 0;
     hipLaunchKernelGGL((k_emit_write<T>), dim3((uint32_t)blocks), dim3(256), 0, s, d_cells,
                        n_cells, thr, vec, d_counts.get<uint32_t>(), d_chunks.get<uint64_t>(),
-                       capacity, d_trip);
+                       capacity, vec_out, d_trip);
     HD_HIP(hipGetLastError());
     HD_HIP(hipStreamSynchronize(s));
   }

@@ -209,6 +209,14 @@ def test_sparse_emit_dev_on_resident_triangle(out_bytes, rng):
         np.testing.assert_array_equal(trip[:, 0], I[sel])
         np.testing.assert_array_equal(trip[:, 1], J[sel])
         np.testing.assert_array_equal(trip[:, 2], want_cells[sel])
+        # an output 4 bytes past a 16-byte boundary (word stores instead of
+        # 16-byte stores) holds the same triplets
+        d_off = torch.zeros(max(count, 1) * 3 + 1, dtype=torch.int32, device="cuda")
+        assert capi.sparse_emit_dev(d_cells, n, out_bytes, thr, d_off[1:], count, stream) == count
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(d_off[1:count * 3 + 1].cpu().numpy().view(np.uint32),
+                                      trip.reshape(-1))
+        assert int(d_off[0]) == 0
         if count > 1:
             with pytest.raises(capi.EngineError, match="capacity"):
                 capi.sparse_emit_dev(d_cells, n, out_bytes, thr, d_trip, count - 1, stream)
