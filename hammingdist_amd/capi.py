@@ -85,6 +85,7 @@ SIGNATURES = {
     "hd_server_result": (C.c_int, [vp, C.c_uint64, vp, C.c_uint64, C.POINTER(C.c_double)]),
     "hd_server_release": (C.c_int, [vp, C.c_uint64]),
     "hd_server_stats": (C.c_int, [vp, u64p, u64p, u64p, u64p, u64p]),
+    "hd_server_shared_stats": (C.c_int, [vp, u64p, u64p]),
     "hd_variant_count": (C.c_int, []),
     "hd_variant_info": (C.c_int, [C.c_int, intp, intp, intp, intp]),
 }
@@ -382,5 +383,9 @@ class Server:
     def stats(self) -> dict:
         v = [C.c_uint64() for _ in range(5)]
         check(lib().hd_server_stats(self._h, *[C.byref(x) for x in v]))
-        return dict(zip(["done", "batches", "overlapped_batches", "max_running",
-                         "device_used"], [x.value for x in v]))
+        out = dict(zip(["done", "batches", "overlapped_batches", "max_running",
+                        "device_used"], [x.value for x in v]))
+        g, q = C.c_uint64(), C.c_uint64()
+        check(lib().hd_server_shared_stats(self._h, C.byref(g), C.byref(q)))
+        out["shared_groups"], out["shared_queries"] = g.value, q.value
+        return out

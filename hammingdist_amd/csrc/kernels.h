@@ -618,15 +618,28 @@ struct RegQuery {
 
   uint2 q[T];
 
+  // NT: non-temporal loads (an operand read once, which should not push
+  // operands read again out of the caches).
+  template <bool NT = false>
   __device__ __forceinline__ void load(const uint2 *__restrict__ query, uint32_t units,
                                        uint32_t lane) {
 #pragma unroll
     for (int t = 0; t < T; ++t) {
       const uint32_t u = lane + kWave * t;
-      q[t] = u < units ? query[u] : make_uint2(~0u, ~0u);
+      if (u >= units) {
+        q[t] = make_uint2(~0u, ~0u);
+      } else if constexpr (NT) {
+        const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(query) + u);
+        q[t] = make_uint2(v.x, v.y);
+      } else {
+        q[t] = query[u];
+      }
     }
   }
 
+  // NT: non-temporal loads (rows read once); false for operands read again
+  // soon (a query streamed against many rows held in registers).
+  template <bool NT = true>
   __device__ __forceinline__ uint32_t distance(const uint2 *__restrict__ row, uint32_t units,
                                                uint32_t lane) const {
     uint2 a[T];
@@ -634,8 +647,12 @@ struct RegQuery {
     for (int t = 0; t < T; ++t) {
       const uint32_t u = lane + kWave * t;
       if (u < units) {
-        const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(row) + u);
-        a[t] = make_uint2(v.x, v.y);
+        if constexpr (NT) {
+          const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(row) + u);
+          a[t] = make_uint2(v.x, v.y);
+        } else {
+          a[t] = row[u];
+        }
       } else {
         a[t] = make_uint2(~0u, ~0u);
       }

@@ -252,6 +252,9 @@ int hd_server_release(hd_server *server, uint64_t request);
 int hd_server_stats(hd_server *server, uint64_t *done, uint64_t *batches,
                     uint64_t *overlapped_batches, uint64_t *max_running,
                     uint64_t *device_used);
+/* Query groups the server ran as shared scans (one read of a database row for
+ * every query of the group that holds it), and the queries in them.         */
+int hd_server_shared_stats(hd_server *server, uint64_t *groups, uint64_t *queries);
 
 /* Number of all-pairs tile shapes compiled in, and a description of one.    */
 int hd_variant_count(void);

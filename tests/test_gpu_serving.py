@@ -69,3 +69,42 @@ def test_oversized_request_is_rejected(db):
     with pytest.raises(capi.EngineError, match="out of range"):
         srv.submit_query(packed[0], np.array([packed.shape[0]], np.uint64))
     srv.close()
+
+
+@pytest.mark.parametrize("min_entries", [None, "1"])
+def test_shared_scan_matches_oracle(db, min_entries, monkeypatch):
+    # Queries of overlapping row sets (rows repeated within a query too),
+    # DENSE and CHAR_X, submitted together so that they batch: groups of a
+    # semantics read each database row once for all of their queries
+    # (HD_SHARED_MIN_ENTRIES=1 takes every group of two or more that way).
+    packed, L = db
+    if min_entries:
+        monkeypatch.setenv("HD_SHARED_MIN_ENTRIES", min_entries)
+    rng = np.random.default_rng(17)
+    n_db = packed.shape[0]
+    hot = rng.choice(n_db, size=300, replace=False)
+    reqs = []
+    for k in range(24):
+        m = int(rng.integers(1, 900))
+        rows = np.concatenate([rng.choice(hot, size=m // 2), rng.integers(0, n_db, m - m // 2)])
+        rng.shuffle(rows)
+        src = int(rng.integers(0, n_db))
+        sem = capi.HD_SEM_DENSE if k % 3 else capi.HD_SEM_CHAR_X
+        reqs.append((src, rows.astype(np.uint64), sem))
+    srv = capi.Server(packed, device_bytes=1 << 30, host_bytes=256 << 20, streams=1,
+                      max_batch=64)
+    ids = [srv.submit_query(packed[src].copy(), rows, sem) for src, rows, sem in reqs]
+    chars = workloads.generate_chars(workloads.large(n=n_db, L=L, seed=77))
+    text = [bytes(c).decode() for c in chars]
+    for rid, (src, rows, sem) in zip(ids, reqs):
+        assert srv.wait(rid) == capi.HD_REQ_DONE
+        got, _ = srv.result(rid)
+        if sem == capi.HD_SEM_DENSE:
+            want = [O.distance_dense(packed[src], packed[int(r)]) for r in rows]
+        else:
+            want = [O.distance_char(text[src], text[int(r)]) for r in rows]
+        np.testing.assert_array_equal(got, np.array(want, np.uint32))
+    st = srv.stats()
+    if min_entries:
+        assert st["shared_groups"] >= 1 and st["shared_queries"] >= 2
+    srv.close()
