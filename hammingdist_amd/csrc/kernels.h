@@ -58,7 +58,7 @@ struct Table16 {
 //                and G_j the '-' sites of row j (padding included).  With the
 //                early exit on, the counts of each kCheckWords-word chunk are
 //                added after that chunk, so every vote sees the exact partial
-//                distance; both come from dense3_counts.
+//                distance; both are written by the encode tiles (encode_tile).
 //   P = kGap2, kGap3  the sparse semantics: a site counts when the classes
 //                differ and neither is '-'.  Every class present and '-'
 //                get distinct 2- or 3-bit codes (planes 0 .. B-1), plane B
