@@ -220,3 +220,41 @@ def test_sparse_emit_dev_on_resident_triangle(out_bytes, rng):
         if count > 1:
             with pytest.raises(capi.EngineError, match="capacity"):
                 capi.sparse_emit_dev(d_cells, n, out_bytes, thr, d_trip, count - 1, stream)
+
+
+_long_rows = {}
+
+
+def long_rows(L):
+    """Three rows of L sites: substitutions, '-' runs, 'N' (shared by variants)."""
+    if L not in _long_rows:
+        rng = np.random.default_rng(5)
+        base = rng.choice(np.frombuffer(b"ACGT", np.uint8), size=L)
+        rows = np.tile(base, (3, 1))
+        for k in range(3):
+            rows[k, rng.integers(0, L, size=1000 * (k + 1))] = \
+                rng.choice(np.frombuffer(b"ACGT-N", np.uint8))
+            rows[k, L // 2 + 7 * k: L // 2 + 7 * k + 5000] = ord("-")
+        _long_rows[L] = (rows, O.pack(rows))
+    return _long_rows[L]
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_longest_rows_of_the_offset_codes(variant):
+    # Rows of 2^18 - 1 words (8,388,576 sites) are the longest the dense
+    # 3-op and sparse gap codes take: their tiles address rows through 31-bit
+    # buffer offsets, which for 8 rows per lane (variant 1) come within 8 KB
+    # of 2^31.  One word more selects the nibble and class codes.
+    present = (1 << 0) | (1 << 1) | (1 << 2) | (1 << 4) | (1 << 8) | (1 << 15)
+    for L, offset_code in [(((1 << 18) - 1) * 32, True), ((1 << 18) * 32, False)]:
+        chars, packed = long_rows(L)
+        n, bps = packed.shape
+        dense_total = capi.planes_layout(n, bps, capi.HD_SEM_DENSE, present)[3]
+        assert (dense_total > 512 * (2 * bps // 32) * 4) == offset_code
+        for md, early in [(2**31 - 1, False), (1500, True)]:
+            want = O.all_pairs_dense(packed, 4, md)
+            got = run_dev(packed, capi.HD_SEM_DENSE, 4, md, early, variant)
+            np.testing.assert_array_equal(got, want, err_msg=f"dense L={L} md={md}")
+            want = O.all_pairs_sparse(chars, False, 4, md)
+            got = run_dev(packed, capi.HD_SEM_SPARSE, 4, md, early, variant)
+            np.testing.assert_array_equal(got, want, err_msg=f"sparse L={L} md={md}")
