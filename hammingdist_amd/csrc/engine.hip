@@ -738,55 +738,92 @@ hipStream_t lib_stream() {
   return s;
 }
 
+// Host-side copy spread over up to 8 threads.
+static void parallel_copy(uint8_t *dst, const uint8_t *src, uint64_t len) {
+  const unsigned threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < threads; ++t)
+    pool.emplace_back([=] {
+      const uint64_t b = len * t / threads, e = len * (t + 1) / threads;
+      std::memcpy(dst + b, src + b, e - b);
+    });
+  std::memcpy(dst, src, len / threads);
+  for (auto &th : pool) th.join();
+}
+
+// Two pinned 64 MiB staging buffers and their events, one set per direction
+// (each guarded by its own mutex: one large copy at a time shares it).  Held
+// until the process ends (never freed, so no HIP call runs during static
+// destruction).
+constexpr uint64_t kStage = 64ull << 20;
+struct Staging {
+  std::mutex mu;
+  PinnedBuffer *buf = nullptr;
+  Event *done = nullptr;
+  void ensure() {
+    if (buf) return;
+    auto *b = new PinnedBuffer[2];
+    b[0].alloc(kStage);
+    b[1].alloc(kStage);
+    done = new Event[2];
+    buf = b;
+  }
+};
+
 // Copies `bytes` of device memory (ordered after the work on `s`) into
-// pageable host memory.  Large copies go through two pinned 64 MiB staging
-// buffers: the DMA of chunk c + 1 overlaps the copy of chunk c out of its
-// staging buffer, which several host threads share (they also take the
-// page faults of a freshly allocated destination in parallel).
+// pageable host memory.  Large copies go through the pinned staging buffers:
+// the DMA of chunk c + 1 overlaps the copy of chunk c out of its staging
+// buffer, which several host threads share (they also take the page faults
+// of a freshly allocated destination in parallel).
 static void download(void *host, const void *dev, uint64_t bytes, hipStream_t s) {
-  constexpr uint64_t kStage = 64ull << 20;
   if (bytes <= kStage) {
     HD_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s));
     HD_HIP(hipStreamSynchronize(s));
     return;
   }
-  static std::mutex mu; // one large download at a time shares the staging
-  std::lock_guard<std::mutex> lock(mu);
-  // Held until the process ends (never freed, so no HIP call runs during
-  // static destruction).
-  static PinnedBuffer *stage = nullptr;
-  static Event *done = nullptr;
-  if (!stage) {
-    auto *st = new PinnedBuffer[2];
-    st[0].alloc(kStage);
-    st[1].alloc(kStage);
-    done = new Event[2];
-    stage = st;
-  }
+  static Staging st;
+  std::lock_guard<std::mutex> lock(st.mu);
+  st.ensure();
   const uint64_t chunks = (bytes + kStage - 1) / kStage;
-  const unsigned threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
   auto enqueue = [&](uint64_t c) {
     const uint64_t off = c * kStage, len = std::min(kStage, bytes - off);
-    HD_HIP(hipMemcpyAsync(stage[c & 1].get(), static_cast<const uint8_t *>(dev) + off, len,
+    HD_HIP(hipMemcpyAsync(st.buf[c & 1].get(), static_cast<const uint8_t *>(dev) + off, len,
                           hipMemcpyDeviceToHost, s));
-    HD_HIP(hipEventRecord(done[c & 1].get(), s));
+    HD_HIP(hipEventRecord(st.done[c & 1].get(), s));
   };
   enqueue(0);
   for (uint64_t c = 0; c < chunks; ++c) {
     if (c + 1 < chunks) enqueue(c + 1); // its buffer was emptied in step c - 1
-    HD_HIP(hipEventSynchronize(done[c & 1].get()));
+    HD_HIP(hipEventSynchronize(st.done[c & 1].get()));
     const uint64_t off = c * kStage, len = std::min(kStage, bytes - off);
-    const uint8_t *src = stage[c & 1].get<uint8_t>();
-    uint8_t *dst = static_cast<uint8_t *>(host) + off;
-    std::vector<std::thread> pool;
-    for (unsigned t = 1; t < threads; ++t)
-      pool.emplace_back([=] {
-        const uint64_t b = len * t / threads, e = len * (t + 1) / threads;
-        std::memcpy(dst + b, src + b, e - b);
-      });
-    std::memcpy(dst, src, len / threads);
-    for (auto &th : pool) th.join();
+    parallel_copy(static_cast<uint8_t *>(host) + off, st.buf[c & 1].get<uint8_t>(), len);
   }
+}
+
+// Copies `bytes` of pageable host memory to the device, ordered on `s`.
+// Large copies go through the pinned staging buffers: several host threads
+// fill chunk c + 1 while the DMA of chunk c runs.  Returns once every chunk
+// has reached the device.
+static void upload(void *dev, const void *host, uint64_t bytes, hipStream_t s) {
+  if (bytes <= kStage) {
+    HD_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s));
+    return;
+  }
+  static Staging st;
+  std::lock_guard<std::mutex> lock(st.mu);
+  st.ensure();
+  const uint64_t chunks = (bytes + kStage - 1) / kStage;
+  for (uint64_t c = 0; c < chunks; ++c) {
+    const int slot = (int)(c & 1);
+    if (c >= 2) HD_HIP(hipEventSynchronize(st.done[slot].get())); // chunk c - 2 has left
+    const uint64_t off = c * kStage, len = std::min(kStage, bytes - off);
+    parallel_copy(st.buf[slot].get<uint8_t>(), static_cast<const uint8_t *>(host) + off, len);
+    HD_HIP(hipMemcpyAsync(static_cast<uint8_t *>(dev) + off, st.buf[slot].get(), len,
+                          hipMemcpyHostToDevice, s));
+    HD_HIP(hipEventRecord(st.done[slot].get(), s));
+  }
+  HD_HIP(hipEventSynchronize(st.done[0].get()));
+  HD_HIP(hipEventSynchronize(st.done[1].get()));
 }
 
 // Uploads n packed rows and builds their planes (plus the nibble mask).
@@ -799,7 +836,7 @@ static void upload_encode(Encoded &e, const uint8_t *packed, uint64_t n,
                           uint64_t bpr, int semantics, hipStream_t s) {
   geometry(n, bpr, HD_SEM_DENSE, 0); // argument checks
   e.packed.alloc(n * bpr);
-  HD_HIP(hipMemcpyAsync(e.packed.get(), packed, n * bpr, hipMemcpyHostToDevice, s));
+  upload(e.packed.get(), packed, n * bpr, s);
   // The nibble mask picks the plane code (also for the dense semantics,
   // whose 3-op code needs the data to hold no X or other codes).
   uint32_t mask = 0xFFFFu;
@@ -1053,7 +1090,7 @@ int hd_reference_distances(const uint8_t *query, const uint8_t *db, uint64_t n_d
     hipStream_t s = lib_stream();
     DeviceBuffer d_q(bytes_per_seq), d_db(n_db * bytes_per_seq), d_out(m * 4), d_rows;
     HD_HIP(hipMemcpyAsync(d_q.get(), query, bytes_per_seq, hipMemcpyHostToDevice, s));
-    HD_HIP(hipMemcpyAsync(d_db.get(), db, n_db * bytes_per_seq, hipMemcpyHostToDevice, s));
+    upload(d_db.get(), db, n_db * bytes_per_seq, s);
     if (rows) {
       d_rows.alloc(m * 8);
       HD_HIP(hipMemcpyAsync(d_rows.get(), rows, m * 8, hipMemcpyHostToDevice, s));
@@ -1134,7 +1171,7 @@ int hd_sparse_emit(const void *cells, uint64_t n, int out_bytes, int64_t thresho
     const uint32_t thr = threshold > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)threshold;
     hipStream_t s = lib_stream();
     DeviceBuffer d_cells(nc * out_bytes);
-    HD_HIP(hipMemcpyAsync(d_cells.get(), cells, nc * out_bytes, hipMemcpyHostToDevice, s));
+    upload(d_cells.get(), cells, nc * out_bytes, s);
     DeviceBuffer d_trip(std::max<uint64_t>(capacity, 1) * 12);
     const uint64_t total =
         emit_dev(d_cells.get(), nc, out_bytes, thr, d_trip.get<uint32_t>(), capacity, s);
