@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -738,6 +739,17 @@ hipStream_t lib_stream() {
   return s;
 }
 
+// A second library stream for downloads that overlap work on lib_stream().
+hipStream_t lib_copy_stream() {
+  static std::once_flag once;
+  static hipStream_t s = nullptr;
+  std::call_once(once, [] {
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+  });
+  if (!s) throw Error(HD_ERR_HIP, "could not create the library copy stream");
+  return s;
+}
+
 // Host-side copy spread over up to 8 threads.
 static void parallel_copy(uint8_t *dst, const uint8_t *src, uint64_t len) {
   const unsigned threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
@@ -770,13 +782,22 @@ struct Staging {
   }
 };
 
-// Copies `bytes` of device memory (ordered after the work on `s`) into
-// pageable host memory.  Large copies go through the pinned staging buffers:
-// the DMA of chunk c + 1 overlaps the copy of chunk c out of its staging
-// buffer, which several host threads share (they also take the page faults
-// of a freshly allocated destination in parallel).
-static void download(void *host, const void *dev, uint64_t bytes, hipStream_t s) {
+// The first `end` bytes of a device buffer are final once `ev` has fired.
+struct ReadyMark {
+  uint64_t end;
+  hipEvent_t ev;
+};
+
+// Copies `bytes` of device memory (ordered after the work on `s`, and, with
+// `marks`, each chunk after the first mark that covers it) into pageable
+// host memory.  Large copies go through the pinned staging buffers: the DMA
+// of chunk c + 1 overlaps the copy of chunk c out of its staging buffer,
+// which several host threads share (they also take the page faults of a
+// freshly allocated destination in parallel).
+static void download(void *host, const void *dev, uint64_t bytes, hipStream_t s,
+                     const std::vector<ReadyMark> *marks = nullptr) {
   if (bytes <= kStage) {
+    if (marks && !marks->empty()) HD_HIP(hipStreamWaitEvent(s, marks->back().ev, 0));
     HD_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s));
     HD_HIP(hipStreamSynchronize(s));
     return;
@@ -785,8 +806,13 @@ static void download(void *host, const void *dev, uint64_t bytes, hipStream_t s)
   std::lock_guard<std::mutex> lock(st.mu);
   st.ensure();
   const uint64_t chunks = (bytes + kStage - 1) / kStage;
+  size_t mark = 0;
   auto enqueue = [&](uint64_t c) {
     const uint64_t off = c * kStage, len = std::min(kStage, bytes - off);
+    if (marks) {
+      while (mark < marks->size() && (*marks)[mark].end < off + len) ++mark;
+      if (mark < marks->size()) HD_HIP(hipStreamWaitEvent(s, (*marks)[mark].ev, 0));
+    }
     HD_HIP(hipMemcpyAsync(st.buf[c & 1].get(), static_cast<const uint8_t *>(dev) + off, len,
                           hipMemcpyDeviceToHost, s));
     HD_HIP(hipEventRecord(st.done[c & 1].get(), s));
@@ -958,9 +984,31 @@ int hd_all_pairs(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
     DeviceBuffer d_out(cells * out_bytes);
     const int v = default_variant(e.g, true, max_dist);
     const uint64_t TI = tile_rows(v);
-    run_allpairs(e.g, e.planes.get<uint32_t>(), out_bytes, max_dist, true, v, 0,
-                 (n + TI - 1) / TI, d_out.get(), 0, s);
-    download(host_out, d_out.get(), cells * out_bytes, s);
+    const uint64_t nb = (n + TI - 1) / TI;
+    // Without early-exit votes (every site compared) the triangle is
+    // computed in row bands of about 1/16 of it (at least one staging
+    // chunk), each launched on its own, so the download of a band's cells
+    // overlaps the computation of the next ones.  With votes the kernel is
+    // short and one launch is faster.
+    const bool votes = check_interval(true, max_dist, e.g.W) < e.g.W;
+    const uint64_t band_bytes =
+        votes ? cells * out_bytes : std::max(kStage, cells * out_bytes / 16);
+    std::vector<std::unique_ptr<Event>> evs;
+    std::vector<ReadyMark> marks;
+    for (uint64_t rb = 0; rb < nb;) {
+      const uint64_t c0 = tri_cells(std::min(n, rb * TI));
+      uint64_t rb1 = rb + 1;
+      while (rb1 < nb && (tri_cells(std::min(n, (rb1 + 1) * TI)) - c0) * out_bytes <= band_bytes)
+        ++rb1;
+      run_allpairs(e.g, e.planes.get<uint32_t>(), out_bytes, max_dist, true, v, rb, rb1,
+                   d_out.get(), 0, s);
+      evs.push_back(std::make_unique<Event>());
+      HD_HIP(hipEventRecord(evs.back()->get(), s));
+      marks.push_back({tri_cells(std::min(n, rb1 * TI)) * out_bytes, evs.back()->get()});
+      rb = rb1;
+    }
+    if (votes) download(host_out, d_out.get(), cells * out_bytes, s);
+    else download(host_out, d_out.get(), cells * out_bytes, lib_copy_stream(), &marks);
   });
 }
 

@@ -192,5 +192,6 @@ void run_allpairs(const Geometry &g, const uint32_t *planes, int out_bytes,
                   uint64_t rb1, void *out, uint64_t cell_offset, hipStream_t s);
 uint32_t check_interval(bool early_exit, uint32_t max_dist, uint32_t W);
 hipStream_t lib_stream();
+hipStream_t lib_copy_stream();
 
 } // namespace hd
