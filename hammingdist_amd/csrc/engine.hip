@@ -985,14 +985,15 @@ int hd_all_pairs(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
     const int v = default_variant(e.g, true, max_dist);
     const uint64_t TI = tile_rows(v);
     const uint64_t nb = (n + TI - 1) / TI;
-    // Without early-exit votes (every site compared) the triangle is
-    // computed in row bands of about 1/16 of it (at least one staging
-    // chunk), each launched on its own, so the download of a band's cells
-    // overlaps the computation of the next ones.  With votes the kernel is
-    // short and one launch is faster.
-    const bool votes = check_interval(true, max_dist, e.g.W) < e.g.W;
+    // The triangle is computed in row bands of about 1/8 of it (at least
+    // one staging chunk), each launched on its own, so the download of a
+    // band's cells overlaps the computation of the next ones.  With early-
+    // exit votes and a small cap (below 64) the kernel is short and one
+    // launch followed by the download is faster (50 vs 57 ms for 50,000 x
+    // 29,903 at max_distance 2; banded 143 vs 159 ms at 255).
+    const bool short_kernel = check_interval(true, max_dist, e.g.W) < e.g.W && max_dist < 64;
     const uint64_t band_bytes =
-        votes ? cells * out_bytes : std::max(kStage, cells * out_bytes / 16);
+        short_kernel ? cells * out_bytes : std::max(kStage, cells * out_bytes / 8);
     std::vector<std::unique_ptr<Event>> evs;
     std::vector<ReadyMark> marks;
     for (uint64_t rb = 0; rb < nb;) {
@@ -1007,7 +1008,7 @@ int hd_all_pairs(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
       marks.push_back({tri_cells(std::min(n, rb1 * TI)) * out_bytes, evs.back()->get()});
       rb = rb1;
     }
-    if (votes) download(host_out, d_out.get(), cells * out_bytes, s);
+    if (short_kernel) download(host_out, d_out.get(), cells * out_bytes, s);
     else download(host_out, d_out.get(), cells * out_bytes, lib_copy_stream(), &marks);
   });
 }
