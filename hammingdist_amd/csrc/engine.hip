@@ -986,14 +986,16 @@ int hd_all_pairs(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
     const uint64_t TI = tile_rows(v);
     const uint64_t nb = (n + TI - 1) / TI;
     // The triangle is computed in row bands of about 1/8 of it (at least
-    // one staging chunk), each launched on its own, so the download of a
+    // one staging chunk, at most 512 MiB, so the transfer left after the
+    // last band stays short), each launched on its own, so the download of a
     // band's cells overlaps the computation of the next ones.  With early-
     // exit votes and a small cap (below 64) the kernel is short and one
     // launch followed by the download is faster (50 vs 57 ms for 50,000 x
     // 29,903 at max_distance 2; banded 143 vs 159 ms at 255).
     const bool short_kernel = check_interval(true, max_dist, e.g.W) < e.g.W && max_dist < 64;
     const uint64_t band_bytes =
-        short_kernel ? cells * out_bytes : std::max(kStage, cells * out_bytes / 8);
+        short_kernel ? cells * out_bytes
+                     : std::max(kStage, std::min<uint64_t>(cells * out_bytes / 8, 8 * kStage));
     std::vector<std::unique_ptr<Event>> evs;
     std::vector<ReadyMark> marks;
     for (uint64_t rb = 0; rb < nb;) {
