@@ -750,9 +750,10 @@ hipStream_t lib_copy_stream() {
   return s;
 }
 
-// Host-side copy spread over up to 8 threads.
+// Host-side copy spread over up to 16 threads (a fresh destination takes
+// its page faults in parallel).
 static void parallel_copy(uint8_t *dst, const uint8_t *src, uint64_t len) {
-  const unsigned threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  const unsigned threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   std::vector<std::thread> pool;
   for (unsigned t = 1; t < threads; ++t)
     pool.emplace_back([=] {

@@ -16,6 +16,13 @@
 
 namespace hdhost {
 
+void advise_huge_pages(void *p, std::size_t bytes) {
+  const std::uintptr_t page = static_cast<std::uintptr_t>(::sysconf(_SC_PAGESIZE));
+  const std::uintptr_t b = (reinterpret_cast<std::uintptr_t>(p) + page - 1) / page * page;
+  const std::uintptr_t e = (reinterpret_cast<std::uintptr_t>(p) + bytes) / page * page;
+  if (e > b) (void)::madvise(reinterpret_cast<void *>(b), e - b, MADV_HUGEPAGE);
+}
+
 namespace {
 
 // Decimal text of 0..65535: digits and length, built once.
