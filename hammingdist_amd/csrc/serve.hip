@@ -243,8 +243,9 @@ __global__ __launch_bounds__(256) void k_sq_scan(const QueryDev *__restrict__ qs
 }
 
 // A shared group holds at least this many row positions (below it the fixed
-// cost of bucketing outweighs the rows it saves) and at most
-// kSharedMaxEntries (8 bytes each in the stream's scratch).
+// cost of bucketing outweighs the rows it saves) and at most the server's
+// shared_cap (8 bytes each in the stream's scratch; kSharedMaxEntries at
+// most).
 constexpr uint64_t kSharedMinEntries = 100000;
 // ... and reads each of its rows for this many queries on average: a shared
 // scan walks every database row and is a single long launch, so groups that
@@ -746,6 +747,9 @@ int hd_server_create(const uint8_t *db, uint64_t n_db, uint64_t bytes_per_seq,
       s->shared_reuse = 0.0;
     }
     if (shared_scan_kernel(HD_SEM_DENSE, bytes_per_seq) && n_db < (1ull << 32)) {
+      // Entries for 64 row positions per database row, within [2^16, 2^24].
+      s->shared_cap = std::min<uint64_t>(kSharedMaxEntries,
+                                         std::max<uint64_t>(1ull << 16, 64 * n_db));
       const uint64_t nb = (n_db + kScanBlock - 1) / kScanBlock;
       for (int k = 0; k < streams; ++k) {
         auto sc = std::make_unique<hd_server::Scratch>();
