@@ -130,9 +130,15 @@ const Variant &variant_of(int v) { return kVariants[v]; }
 int default_variant(const Geometry &g, bool early_exit, uint32_t max_dist) {
   const uint64_t n = g.n;
   if (n < 512) return 4;
-  // With early-exit votes the kGap codes' kernel runs faster with 8 columns
-  // per wavefront (fewer SGPRs to hold across each vote).
+  // A cap below kSmallCap is usually passed within the first chunks: a tile
+  // stops at the vote its slowest pair passes, and 128 x 32 tiles stop
+  // sooner than larger ones (n = 50,000, L = 29,903, cap 2: 7.6 ms against
+  // 9.2 ms for 256 x 64).  With larger caps and votes the kGap codes'
+  // kernel runs faster with 8 columns per wavefront (fewer SGPRs to hold
+  // across each vote).
+  constexpr uint32_t kSmallCap = 64;
   const bool votes = check_interval(early_exit, max_dist, g.W) < g.W;
+  if (votes && max_dist < kSmallCap) return 4;
   if (votes && (g.code.op == kGap2 || g.code.op == kGap3)) return 2;
   if (g.code.planes <= 4) return n < 2048 ? 3 : 0;
   return 2;

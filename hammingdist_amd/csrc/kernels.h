@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "hammingdist_amd.h"
 
@@ -67,10 +68,13 @@ struct Table16 {
 //                (3 or 4 ops), f = codes differ.  Per site f - [row i '-'] -
 //                [row j '-'] + 2 [both '-'] is the site's distance term, so
 //                popcount(f) - G_i - G_j + 2 |g_i & g_j| is the distance,
-//                where G counts the '-' sites of a row (padding included).
-//                The last term is summed over the words where a column has
-//                a '-' only: a per-row bitmap of such words (gap_corr_words)
-//                drives a wave-uniform loop after the main one.  With the
+//                where G counts the '-' sites of a row.  Sites past the
+//                packed row are '-' in every row and add nothing to any
+//                pair: they keep the '-' code but stay out of plane B and
+//                of G.  The last term is summed over the words where a
+//                column has a '-' only: a per-row bitmap of such words and
+//                a summary of its chunks (gap_corr_words) drive a
+//                wave-uniform loop after the main one.  With the
 //                early exit on, chunk counts of G make the votes use
 //                popcount - G_i - G_j, a lower bound of the partial distance.
 //   P = kClass3  the sparse semantics for rows of kBufWords words or more: 3
@@ -101,11 +105,17 @@ __host__ __device__ constexpr uint64_t dense3_corr_words(uint32_t W, uint64_t n_
 }
 
 // kGap2 / kGap3 correction words after the planes (same size as kDense3's):
-// G[n_pad] (whole rows, twice), then Gc[T][n_pad] per chunk of kCheckWords
-// words and BM[T][n_pad], bit k of BM[t][row] set when word 32 t + k of the
-// row holds a '-'.
+// G[n_pad] (whole rows), S[n_pad], then Gc[T][n_pad] per chunk of
+// kCheckWords words and BM[T][n_pad], bit k of BM[t][row] set when word
+// 32 t + k of the row holds a '-'.  Bit b of S[row] is set when one of the
+// chunks [b s, (b + 1) s) has a bit of BM set, s = gap_summary_span(T).
 __host__ __device__ constexpr uint64_t gap_corr_words(uint32_t W, uint64_t n_pad) {
   return dense3_corr_words(W, n_pad);
+}
+
+// Chunks of kCheckWords words per bit of the kGap codes' summary word S.
+__host__ __device__ constexpr uint32_t gap_summary_span(uint32_t T) {
+  return (T + 31) / 32;
 }
 
 // Planes stored per row and word for plane code P.
@@ -129,21 +139,99 @@ __device__ __forceinline__ void tile_coords(uint64_t t, uint32_t M, uint32_t &b,
   cb = (uint32_t)(t - (uint64_t)M * r * (r + 1) / 2);
 }
 
+// LDS bytes each wave of a staged-store kernel owns.
+constexpr int kStageBytes = 4096;
+
+// Orders this wave's LDS writes before its later LDS reads (the LDS serves
+// one wave's accesses in order; the fences keep the compiler from moving
+// them).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Stores one wave's R x C cells, v[q][c] for row i0 + 64 q + lane and column
+// j0 + c (already clamped), as SB-byte cells of the triangle.  Stored
+// straight from the registers, each store instruction writes 64 rows, one
+// cell each: 64 cache lines.  Here the cells go through the wave's LDS
+// (kStageBytes at `stage`, QP row groups of 64 at a time) and each store
+// instruction writes 64 / C whole row pieces of C consecutive cells.
+template <int R, int C, int SB>
+__device__ __forceinline__ void tile_store_staged(const uint32_t (&v)[R][C], uint32_t n,
+                                                  uint32_t i0, uint32_t j0, void *out,
+                                                  uint64_t cell_offset, uint8_t *stage) {
+  using T = typename std::conditional<
+      SB == 1, uint8_t, typename std::conditional<SB == 2, uint16_t, uint32_t>::type>::type;
+  constexpr int RB = C * SB;                 // bytes of one row piece
+  constexpr int QB = kWave * RB;             // bytes of 64 row pieces
+  constexpr int QP = QB * R <= kStageBytes ? R : kStageBytes / QB;
+  constexpr int RS = kWave / C;              // row pieces per store instruction
+  static_assert(QP >= 1 && R % QP == 0, "stage must hold whole row groups");
+  static_assert(RB % 8 == 0 && kWave % C == 0, "row pieces of 8-byte multiples");
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  T *const st = reinterpret_cast<T *>(stage);
+  T *const dst = static_cast<T *>(out);
+  const uint32_t sub = lane / C, cl = lane % C;
+#pragma unroll
+  for (int q0 = 0; q0 < R; q0 += QP) {
+    if (q0) wave_lds_sync(); // the previous pass's reads come first
+#pragma unroll
+    for (int dq = 0; dq < QP; ++dq) {
+      uint32_t w[RB / 4];
+#pragma unroll
+      for (int k = 0; k < RB / 4; ++k) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int e = 0; e < 4 / SB; ++e)
+          x |= (uint32_t)(T)v[q0 + dq][k * (4 / SB) + e] << (8 * SB * e);
+        w[k] = x;
+      }
+      uint8_t *row = stage + (dq * kWave + lane) * RB;
+      if constexpr (RB % 16 == 0) {
+#pragma unroll
+        for (int k = 0; k < RB / 16; ++k)
+          reinterpret_cast<uint4 *>(row)[k] =
+              make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < RB / 8; ++k)
+          reinterpret_cast<uint2 *>(row)[k] = make_uint2(w[2 * k], w[2 * k + 1]);
+      }
+    }
+    wave_lds_sync();
+    // Store s: row piece RS s + sub of this pass, cell cl.  Row i's first
+    // cell is i (i - 1) / 2; RS rows further on it is RS i + RS (RS - 1) / 2
+    // cells further.
+    uint32_t i = i0 + kWave * q0 + sub;
+    uint64_t x = (uint64_t)i * (i - 1) / 2 + j0 + cl - cell_offset;
+#pragma unroll
+    for (int s = 0; s < QP * C; ++s) {
+      const T val = st[s * kWave + lane];
+      if (i < n && j0 + cl < i) dst[x] = val;
+      x += (uint64_t)RS * i + RS * (RS - 1) / 2;
+      i += RS;
+    }
+  }
+}
+
 // One wave's share of a tile: rows i0 + 64 q + lane (q < R) against columns
 // j0 .. j0 + C - 1, stored into the triangle at cell i(i-1)/2 + j - cell_offset
 // as out_bytes-wide cells (1, 2 or 4; a constant where the caller's is).
 // corr (codes with has_corr only): the correction counts of the planes' rows
 // (dense3_corr_words / gap_corr_words); check_every is W or kCheckWords.
 // EE = false compiles the loop without early-exit votes (check_every is then
-// ignored).
-template <int P, int R, int C, bool EE = true>
+// ignored).  SB = out_bytes (1, 2 or 4) stores the cells through `stage`,
+// kStageBytes of LDS owned by this wave (tile_store_staged); SB = 0 stores
+// them straight from the accumulators.
+template <int P, int R, int C, bool EE = true, int SB = 0>
 __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, uint32_t n,
                                           uint32_t W, uint32_t i0, uint32_t j0,
                                           void *__restrict__ out, uint32_t out_bytes,
                                           uint64_t cell_offset, uint32_t max_dist,
                                           uint32_t check_every,
                                           const uint32_t *__restrict__ corr,
-                                          uint32_t n_pad) {
+                                          uint32_t n_pad, uint8_t *stage = nullptr) {
   constexpr bool D3 = P == kDense3;
   constexpr bool GAP = P == kGap2 || P == kGap3;
   constexpr bool CLS = P == kClass3;
@@ -197,7 +285,7 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
     // No votes: the whole-row counts go in at the start.
     uint32_t gj[C];
 #pragma unroll
-    for (int c = 0; c < C; ++c) gj[c] = corr[n_pad + j0 + c];
+    for (int c = 0; c < C; ++c) gj[c] = corr[(GAP ? 0 : n_pad) + j0 + c];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const uint32_t z = corr[min(i0 + kWave * q + lane, n_pad - 1)];
@@ -313,41 +401,59 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   if constexpr (OFFSET && EE) {
     if (check_every >= W) { // one chunk: the whole-row counts
       __asm__ volatile("" ::: "memory");
-      add_counts(corr, corr + n_pad);
+      add_counts(corr, GAP ? corr : corr + n_pad);
     }
   }
   if constexpr (GAP) {
     // + 2 |g_i & g_j| over the words where column j has a '-': per column, a
-    // wave-uniform loop over the set bits of its bitmap.  After an early exit
-    // every real pair's count is already a lower bound >= md, and adding to
-    // it keeps it there.
+    // wave-uniform loop over the chunks its summary word marks and the set
+    // bits of their bitmap words.  After an early exit every real pair's
+    // count is already a lower bound >= md, and adding to it keeps it there.
     __asm__ volatile("" ::: "memory");
     const uint32_t T = (W + kCheckWords - 1) / kCheckWords;
+    const uint32_t span = gap_summary_span(T);
     const cptr bm = (cptr)(corr + (2 + (uint64_t)T) * n_pad);
+    const cptr summary = (cptr)(corr + n_pad);
+    uint32_t sm[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) sm[c] = summary[j0 + c];
     // The rows' base address is recomputed from a lane id the compiler
     // cannot tie to `lane`, so no copy of it stays live across the main loop.
     const uint32_t lane_b = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
     const cptr pr = (cptr)(planes + (size_t)(i0 / kWave) * group + lane_b);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      for (uint32_t t = 0; t < (j0 + c < n ? T : 0u); ++t) {
-        uint32_t u = bm[(uint64_t)t * n_pad + j0 + c];
-        while (u) {
-          const uint32_t w = t * kCheckWords + (uint32_t)__builtin_ctz(u);
-          u &= u - 1;
-          const uint64_t step = ((uint64_t)w * PS + GB) * kWave;
-          const uint32_t gb = pb0[step + c];
+      uint32_t chunks = j0 + c < n ? sm[c] : 0u;
+      while (chunks) {
+        const uint32_t t0 = (uint32_t)__builtin_ctz(chunks) * span;
+        chunks &= chunks - 1;
+        for (uint32_t t = t0; t < min(T, t0 + span); ++t) {
+          uint32_t u = bm[(uint64_t)t * n_pad + j0 + c];
+          while (u) {
+            const uint32_t w = t * kCheckWords + (uint32_t)__builtin_ctz(u);
+            u &= u - 1;
+            const uint64_t step = ((uint64_t)w * PS + GB) * kWave;
+            const uint32_t gb = pb0[step + c];
 #pragma unroll
-          for (int q = 0; q < R; ++q) {
-            const uint32_t x = pr[q * group + step] & gb;
-            acc[q][c] += __builtin_popcount(x);
-            acc[q][c] += __builtin_popcount(x);
+            for (int q = 0; q < R; ++q) {
+              const uint32_t x = pr[q * group + step] & gb;
+              acc[q][c] += __builtin_popcount(x);
+              acc[q][c] += __builtin_popcount(x);
+            }
           }
         }
       }
     }
   }
 
+  if constexpr (SB != 0) {
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc[q][c] = min(acc[q][c] - OFF, max_dist);
+    tile_store_staged<R, C, SB>(acc, n, i0, j0, out, cell_offset, stage);
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const uint32_t i = i0 + kWave * q + lane;
@@ -385,8 +491,18 @@ __device__ __forceinline__ void allpairs_tile(
   uint32_t b, cb;
   tile_coords(t, TI / TJ, b, cb);
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  tile_wave<P, R, C, EE>(planes, n, W, b * TI, cb * TJ + wave * C, out, sizeof(OutT),
-                         cell_offset, max_dist, check_every, corr, n_pad);
+  if constexpr (EE) {
+    // Early-exit tiles stop after a few chunks, and their stores weigh: they
+    // go through LDS.  Tiles that run every word store straight from the
+    // registers (measured as fast, with no LDS held).
+    __shared__ __attribute__((aligned(16))) uint8_t stage[NW][kStageBytes];
+    tile_wave<P, R, C, EE, (int)sizeof(OutT)>(planes, n, W, b * TI, cb * TJ + wave * C, out,
+                                              sizeof(OutT), cell_offset, max_dist,
+                                              check_every, corr, n_pad, stage[wave]);
+  } else {
+    tile_wave<P, R, C, EE>(planes, n, W, b * TI, cb * TJ + wave * C, out, sizeof(OutT),
+                           cell_offset, max_dist, check_every, corr, n_pad);
+  }
 }
 
 template <int P, int R, int C, int NW, typename OutT, bool EE>
@@ -422,7 +538,8 @@ __global__ __launch_bounds__(kWave *NW, 4) void k_allpairs_gap(
 // correction counts the tile also writes its chunk's counts (the layout of
 // dense3_corr_words / gap_corr_words); the whole-row slots are summed
 // afterwards (corr_totals_row).  Rows >= n and sites past the row end
-// encode as '-' (nibble 15).
+// encode as '-' (nibble 15); the kGap codes keep the latter out of their
+// '-' plane.
 constexpr int kEncodeWords = 32;
 // LDS row stride in bytes: 16 bytes of padding make lanes reading the same
 // word of consecutive rows fall into different banks.
@@ -517,6 +634,12 @@ __device__ __forceinline__ void encode_tile(const uint8_t *__restrict__ packed, 
     const uint32_t bytes[4] = {v.x, v.y, v.z, v.w};
     uint32_t w[6];
     encode_word(bytes, P, s_tbl, mode, w);
+    if (!d3 && corr) {
+      // kGap*: sites past the packed row leave the '-' plane (their '-'
+      // code, the same in every row, stays).
+      const uint64_t first = (uint64_t)k * 32, sites = 2 * bpr;
+      if (first + 32 > sites) w[gp] &= first < sites ? (1u << (sites - first)) - 1u : 0u;
+    }
     uint32_t *dst = planes + (((uint64_t)g * W + k) * P) * kWave + r;
 #pragma unroll
     for (int p = 0; p < 6; ++p)
@@ -551,17 +674,20 @@ __device__ __forceinline__ void encode_tile(const uint8_t *__restrict__ packed, 
 }
 
 // Whole-row slots of the correction counts of `row` from its chunk counts:
-// kDense3 Z and G, kGap* G twice.
+// kDense3 Z and G, kGap* G and the chunk summary S of its bitmap.
 __device__ __forceinline__ void corr_totals_row(uint32_t *__restrict__ corr, uint32_t W,
                                                 uint32_t n_pad, int op, uint32_t row) {
   const uint32_t T = (W + kCheckWords - 1) / kCheckWords;
-  uint32_t a = 0, b = 0;
+  const uint32_t span = gap_summary_span(T);
+  uint32_t a = 0, b = 0, s = 0;
   for (uint32_t t = 0; t < T; ++t) {
     a += corr[(2 + (uint64_t)t) * n_pad + row];
-    b += corr[(2 + (uint64_t)T + t) * n_pad + row];
+    const uint32_t v = corr[(2 + (uint64_t)T + t) * n_pad + row];
+    b += v;
+    s |= (v != 0 ? 1u : 0u) << (t / span);
   }
   corr[row] = a;
-  corr[n_pad + row] = op == kDense3 ? b : a;
+  corr[n_pad + row] = op == kDense3 ? b : s;
 }
 
 // Number of tiles of the rows blocks [0, rb) for an M column blocks per row
