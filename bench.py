@@ -152,11 +152,11 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def measured_traffic(n: int, L: int, md: int):
+def measured_traffic(n: int, L: int, md: int, rows_per_lane: int, cols_per_wave: int):
     """HBM bytes per launch of the headline kernel from the committed
     rocprofv3 counter summary of the same command (FETCH_SIZE and WRITE_SIZE
     per dispatch, each scaled by the same counter's reading for a copy of
-    known size); None for any other configuration."""
+    known size); None for any other configuration or tile shape."""
     path = os.path.join(ROOT, "profiles", "k_allpairs_dense_large50k.json")
     try:
         with open(path) as f:
@@ -164,6 +164,8 @@ def measured_traffic(n: int, L: int, md: int):
     except (OSError, ValueError):
         return None
     if (p.get("n"), p.get("L"), p.get("max_distance")) != (n, L, md):
+        return None
+    if f"<3, {rows_per_lane}, {cols_per_wave}, 4," not in p.get("kernel", ""):
         return None
     return p.get("traffic_bytes")
 
@@ -348,6 +350,13 @@ def run_large(args, torch, capi, workloads, np):
     site_cmp = cells * args.L
     ops = cells * W * (OPS_PER_WORD_DENSE3 if dense3 else OPS_PER_WORD[P])
     achieved = ops / (kern_ms / 1e3)
+    variant = (args.variant if args.variant >= 0
+               else capi.default_variant(n, bps, sem, mask, False, md))
+    shape = capi.variants()[variant]
+    R, Cw = shape["rows_per_lane"], shape["cols_per_wave"]
+    kernel_name = (f"k_allpairs<{3 if dense3 else P},{R},{Cw},4,uint8> "
+                   f"({'dense 3-op code' if dense3 else 'dense, 4 planes'}, "
+                   f"{64 * R} x {4 * Cw} tiles)")
     res = {
         "value": site_cmp / (ms_per_step / 1e3),
         "ms_per_step": ms_per_step,
@@ -357,7 +366,7 @@ def run_large(args, torch, capi, workloads, np):
             "n": n, "L": args.L, "cells": cells, "out": "uint8", "max_distance": md,
             "semantics": "dense", "planes": P, "words_per_row": W,
             "plane_code": "dense 3-op" if dense3 else "nibble",
-            "early_exit": False, "variant": args.variant,
+            "early_exit": False, "variant": variant,
         },
         "roofline": {
             "bound": "valu",
@@ -365,9 +374,8 @@ def run_large(args, torch, capi, workloads, np):
             "peak": PEAK_INT32_OPS / 1e12,
             "unit": "TOP/s",
             "frac": achieved / PEAK_INT32_OPS,
-            "traffic": measured_traffic(n, args.L, md),
-            "kernel": ("k_allpairs<3,4,16,4,uint8> (dense 3-op code)" if dense3
-                       else "k_allpairs<4,4,16,4,uint8> (dense, 4 planes)"),
+            "traffic": measured_traffic(n, args.L, md, R, Cw),
+            "kernel": kernel_name,
             "kernel_ms": kern_ms,
             "ops_per_launch": ops,
         },

@@ -88,6 +88,8 @@ SIGNATURES = {
     "hd_server_shared_stats": (C.c_int, [vp, u64p, u64p]),
     "hd_variant_count": (C.c_int, []),
     "hd_variant_info": (C.c_int, [C.c_int, intp, intp, intp, intp]),
+    "hd_default_variant": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, C.c_int,
+                                     C.c_uint32, intp]),
 }
 
 
@@ -209,6 +211,15 @@ def planes_layout(n: int, bps: int, semantics: int, mask: int):
     check(lib().hd_planes_layout(n, bps, semantics, mask, C.byref(n_pad), C.byref(W),
                                  C.byref(P), C.byref(total)))
     return n_pad.value, W.value, P.value, total.value
+
+
+def default_variant(n: int, bps: int, semantics: int, mask: int, early_exit: bool,
+                    max_dist: int) -> int:
+    """The tile shape variant -1 selects for these inputs."""
+    v = C.c_int()
+    check(lib().hd_default_variant(n, bps, semantics, mask, 1 if early_exit else 0, max_dist,
+                                   C.byref(v)))
+    return v.value
 
 
 def nibble_mask_dev(d_packed, n: int, bps: int, stream: int = 0) -> int:

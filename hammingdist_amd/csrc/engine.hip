@@ -133,13 +133,18 @@ int default_variant(const Geometry &g, bool early_exit, uint32_t max_dist) {
   // A cap below kSmallCap is usually passed within the first chunks: a tile
   // stops at the vote its slowest pair passes, and 128 x 32 tiles stop
   // sooner than larger ones (n = 50,000, L = 29,903, cap 2: 7.6 ms against
-  // 9.2 ms for 256 x 64).  With larger caps and votes the kGap codes'
-  // kernel runs faster with 8 columns per wavefront (fewer SGPRs to hold
-  // across each vote).
+  // 9.2 ms for 256 x 64).
   constexpr uint32_t kSmallCap = 64;
   const bool votes = check_interval(early_exit, max_dist, g.W) < g.W;
   if (votes && max_dist < kSmallCap) return 4;
-  if (votes && (g.code.op == kGap2 || g.code.op == kGap3)) return 2;
+  // The kGap codes' kernel runs 256 x 32 tiles at 6 waves per SIMD (77
+  // VGPRs): at n = 100,000, L = 10,000 without votes 144 ms, against 159 ms
+  // for 256 x 64 tiles at 5 waves.
+  if (g.code.op == kGap2 || g.code.op == kGap3) return 2;
+  // Dense codes: 256 x 64 tiles.  128 x 64 tiles (8 waves per SIMD) ran
+  // 0.7 % faster at n = 50,000, L = 29,903 but fetch twice the bytes from
+  // HBM (240 GB against 124 GB per launch: each tile streams its columns
+  // over the whole row, for half as many rows).
   if (g.code.planes <= 4) return n < 2048 ? 3 : 0;
   return 2;
 }
@@ -933,6 +938,16 @@ int hd_planes_layout(uint64_t n, uint64_t bytes_per_seq, int semantics,
     if (n_words) *n_words = g.W;
     if (n_planes) *n_planes = (uint32_t)g.code.planes;
     if (total_words) *total_words = g.total_words();
+  });
+}
+
+int hd_default_variant(uint64_t n, uint64_t bytes_per_seq, int semantics,
+                       uint32_t nibble_mask, int early_exit, uint32_t max_dist,
+                       int *variant) {
+  return guarded([&] {
+    HD_REQUIRE(variant, "variant pointer required");
+    const Geometry g = geometry(n, bytes_per_seq, semantics, nibble_mask);
+    *variant = default_variant(g, early_exit != 0, max_dist);
   });
 }
 

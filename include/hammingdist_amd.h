@@ -260,6 +260,11 @@ int hd_server_shared_stats(hd_server *server, uint64_t *groups, uint64_t *querie
 int hd_variant_count(void);
 int hd_variant_info(int variant, int *rows_per_lane, int *cols_per_wave,
                     int *waves, int *planes);
+/* The tile shape variant = -1 selects for this dataset, cap and early-exit
+ * flag (hd_all_pairs, hd_all_pairs_dev and the other all-pairs calls).     */
+int hd_default_variant(uint64_t n, uint64_t bytes_per_seq, int semantics,
+                       uint32_t nibble_mask, int early_exit, uint32_t max_dist,
+                       int *variant);
 
 #ifdef __cplusplus
 }

@@ -102,3 +102,29 @@ def test_long_row_sparse_code_layout(monkeypatch):
     monkeypatch.setenv("HD_SPARSE_CLASS_CODE", "1")
     n_pad, W, P, total = capi.planes_layout(1000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF)
     assert (n_pad, W, P) == (1024, 313, 4) and total == 1024 * 313 * 4
+
+
+def test_default_tile_shape():
+    from hammingdist_amd import capi
+
+    shapes = {v["variant"]: (v["rows_per_lane"], v["cols_per_wave"]) for v in capi.variants()}
+    acgt_gap_inv = (1 << 0) | (1 << 1) | (1 << 2) | (1 << 4) | (1 << 8) | (1 << 15)
+
+    def shape(n, bps, sem, mask, ee, md):
+        return shapes[capi.default_variant(n, bps, sem, mask, ee, md)]
+
+    # small datasets: 128 x 32 tiles
+    assert shape(300, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 255) == (2, 8)
+    # dense 3-op code comparing every site: 256 x 64 tiles
+    assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 2) == (4, 16)
+    # early exit under a small cap: 128 x 32 tiles stop soonest
+    assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 2) == (2, 8)
+    # early exit with a cap of 255: 256 x 64 tiles
+    assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 255) == (4, 16)
+    # a cap out of reach of the row length turns the votes off
+    assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 65535) == (4, 16)
+    assert shape(1000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 255) == (2, 16)
+    # gap codes: 256 x 32 tiles, with and without votes
+    assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, False, 65535) == (4, 8)
+    assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, True, 255) == (4, 8)
+    assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, True, 3) == (2, 8)
