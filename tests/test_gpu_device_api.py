@@ -37,7 +37,9 @@ def test_every_variant_matches_oracle(variant, alphabet, include_x, rng):
     for n, L in [(2, 3), (65, 64), (600, 300), (1100, 97)]:
         seqs = random_strings(rng, n, L, alphabet)
         packed = O.pack(O.as_chars(seqs), include_x)
-        for out_bytes, md in [(1, 255), (2, 7), (4, 2**32 - 1)]:
+        # (4, 9): 32-bit cells under early-exit votes, stored through LDS
+        # one row group of 64 at a time
+        for out_bytes, md in [(1, 255), (2, 7), (4, 2**32 - 1), (4, 9)]:
             want = O.all_pairs_dense(packed, out_bytes, md)
             for early in (False, True):
                 got = run_dev(packed, capi.HD_SEM_DENSE, out_bytes, md, early, variant)
@@ -75,7 +77,7 @@ def gap_heavy_rows(rng, n, L, alphabet):
 
 
 GAP_CASES = [(600, 1290, "ACGTNX"), (300, 2100, "ACG"), (70, 33, "AC")]
-GAP_CAPS = [(2, 65535), (1, 3), (4, 2**32 - 1), (2, 40), (1, 0)]
+GAP_CAPS = [(2, 65535), (1, 3), (4, 2**32 - 1), (2, 40), (1, 0), (4, 50)]
 _gap_oracle = {}
 
 
