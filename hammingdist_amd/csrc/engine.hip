@@ -141,12 +141,14 @@ int default_variant(const Geometry &g, bool early_exit, uint32_t max_dist) {
   // VGPRs): at n = 100,000, L = 10,000 without votes 144 ms, against 159 ms
   // for 256 x 64 tiles at 5 waves.
   if (g.code.op == kGap2 || g.code.op == kGap3) return 2;
-  // Dense codes: 256 x 64 tiles.  128 x 64 tiles (8 waves per SIMD) ran
-  // 0.7 % faster at n = 50,000, L = 29,903 but fetch twice the bytes from
-  // HBM (240 GB against 124 GB per launch: each tile streams its columns
-  // over the whole row, for half as many rows).
-  if (g.code.planes <= 4) return n < 2048 ? 3 : 0;
-  return 2;
+  // The dense 3-op code: 256 x 64 tiles.  128 x 64 tiles (8 waves per SIMD)
+  // ran 0.7 % faster at n = 50,000, L = 29,903 but fetch twice the bytes
+  // from HBM (240 GB against 124 GB per launch: each tile streams its
+  // columns over the whole row, for half as many rows).
+  if (g.code.op == kDense3) return n < 2048 ? 3 : 0;
+  // The pointer-addressed codes (nibble, one-hot, class-3): 128 x 64 tiles,
+  // 4.5-8 % faster than 256 x 64 and 256 x 32 at n = 30,000, L = 10,000.
+  return 3;
 }
 
 template <int P, int R, int C, int NW, typename OutT>
@@ -168,9 +170,14 @@ static void launch_allpairs_t(const uint32_t *planes, uint32_t n,
   if constexpr (P == kGap2 || P == kGap3)
     kernel = check_every < W ? k_allpairs_gap<P, R, C, NW, OutT, true>
                              : k_allpairs_gap<P, R, C, NW, OutT, false>;
-  else
+  else if constexpr (P == kDense3)
     kernel = check_every < W ? k_allpairs<P, R, C, NW, OutT, true>
                              : k_allpairs<P, R, C, NW, OutT, false>;
+  else
+    // The pointer-addressed codes take the vote-compiled kernel also when
+    // there is no vote (check_every = W: one chunk, the vote skipped): their
+    // vote-free builds spill 18-128 SGPRs into VGPR lanes.
+    kernel = k_allpairs<P, R, C, NW, OutT, true>;
   hipLaunchKernelGGL(kernel, dim3((uint32_t)nt), dim3(64 * NW), 0, s, planes, n, W, t0,
                      (uint32_t)nt, out, cell_offset, max_dist, check_every, corr, n_pad);
   HD_HIP(hipGetLastError());

@@ -128,3 +128,7 @@ def test_default_tile_shape():
     assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, False, 65535) == (4, 8)
     assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, True, 255) == (4, 8)
     assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, True, 3) == (2, 8)
+    # the nibble code (dense data holding X) and the one-hot code: 128 x 64
+    assert shape(30000, 5000, capi.HD_SEM_DENSE, acgt_gap_inv | (1 << 3), False, 65535) == \
+        (2, 16)
+    assert shape(30000, 5000, capi.HD_SEM_CHAR_X, 0xFFFF, False, 65535) == (2, 16)
