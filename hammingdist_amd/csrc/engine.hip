@@ -532,6 +532,7 @@ static void launch_refdist(const uint8_t *d_query, const uint8_t *d_db,
 static constexpr uint32_t kEmitPerThread = 16;
 static constexpr uint32_t kEmitCells = 256 * kEmitPerThread;
 static constexpr uint32_t kScanChunk = 256 * 16; // block counts per scan block
+static constexpr uint32_t kEmitRounds = 2;        // LDS turns of k_emit_write
 
 // The 16 cells x0 .. x0+15 as uint32 values; returns how many of them
 // exist (cells past n_cells read as 0 and must be ignored).  vec: x0 is
@@ -666,48 +667,63 @@ __global__ __launch_bounds__(256) void k_emit_write(const T *__restrict__ cells,
     }
   // The block's triplets are contiguous in the output: they are assembled in
   // LDS in cell order, then stored with consecutive lanes on consecutive
-  // words.
-  __shared__ uint32_t s_trip[3 * kEmitCells];
+  // words.  The block's threads take kEmitRounds turns (the cells of
+  // 256 / kEmitRounds threads each), which keeps the LDS per workgroup at
+  // 3 * 4 * kEmitCells / kEmitRounds bytes.
+  __shared__ uint32_t s_trip[3 * kEmitCells / kEmitRounds];
+  __shared__ uint32_t s_round[kEmitRounds + 1];
+  constexpr uint32_t kRoundThreads = 256 / kEmitRounds;
   uint64_t total = 0;
-  uint32_t at = (uint32_t)block_exclusive_scan(hits, &total);
+  const uint32_t at = (uint32_t)block_exclusive_scan(hits, &total);
+  if (threadIdx.x % kRoundThreads == 0) s_round[threadIdx.x / kRoundThreads] = at;
+  if (threadIdx.x == 0) s_round[kEmitRounds] = (uint32_t)total;
   const uint64_t base = local_prefix[blockIdx.x] + chunk_prefix[blockIdx.x / kScanChunk];
+  const uint64_t limit = 3 * capacity;
+  uint64_t i = 0, row_start = 0;
   if (mask) {
-    uint64_t i = row_of_cell(x0);
-    uint64_t row_start = i * (i - 1) / 2;
+    i = row_of_cell(x0);
+    row_start = i * (i - 1) / 2;
+  }
+  for (uint32_t r = 0; r < kEmitRounds; ++r) {
+    __syncthreads(); // s_round is written; the previous round's stores read s_trip
+    const uint32_t r0 = s_round[r], r1 = s_round[r + 1];
+    if (threadIdx.x / kRoundThreads == r && mask) {
+      uint32_t a = at - r0;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const uint64_t x = x0 + e;
-      while (x >= row_start + i) { // next row (rows of length i)
-        row_start += i;
-        ++i;
-      }
-      if ((mask >> e) & 1u) {
-        s_trip[3 * at + 0] = (uint32_t)i;
-        s_trip[3 * at + 1] = (uint32_t)(x - row_start);
-        s_trip[3 * at + 2] = v[e];
-        ++at;
+      for (int e = 0; e < 16; ++e) {
+        const uint64_t x = x0 + e;
+        while (x >= row_start + i) { // next row (rows of length i)
+          row_start += i;
+          ++i;
+        }
+        if ((mask >> e) & 1u) {
+          s_trip[3 * a + 0] = (uint32_t)i;
+          s_trip[3 * a + 1] = (uint32_t)(x - row_start);
+          s_trip[3 * a + 2] = v[e];
+          ++a;
+        }
       }
     }
+    __syncthreads();
+    const uint64_t first = 3 * (base + r0);
+    const uint64_t words = first >= limit ? 0 : min<uint64_t>(3ull * (r1 - r0), limit - first);
+    if (!vec_out) {
+      for (uint64_t w = threadIdx.x; w < words; w += 256) trip[first + w] = s_trip[w];
+      continue;
+    }
+    // 16-byte stores from the first 16-byte boundary of the output on (the
+    // output itself is 16-byte aligned); single words before and after.
+    const uint32_t head = (uint32_t)min<uint64_t>((4u - (uint32_t)(first & 3u)) & 3u, words);
+    const uint64_t quads = (words - head) / 4;
+    if (threadIdx.x < head) trip[first + threadIdx.x] = s_trip[threadIdx.x];
+    uint4 *dst = reinterpret_cast<uint4 *>(trip + first + head);
+    for (uint64_t q = threadIdx.x; q < quads; q += 256) {
+      const uint32_t *src = s_trip + head + 4 * q;
+      dst[q] = make_uint4(src[0], src[1], src[2], src[3]);
+    }
+    for (uint64_t w = head + 4 * quads + threadIdx.x; w < words; w += 256)
+      trip[first + w] = s_trip[w];
   }
-  __syncthreads();
-  const uint64_t first = 3 * base, limit = 3 * capacity;
-  const uint64_t words = first >= limit ? 0 : min(3 * total, limit - first);
-  if (!vec_out) {
-    for (uint64_t w = threadIdx.x; w < words; w += 256) trip[first + w] = s_trip[w];
-    return;
-  }
-  // 16-byte stores from the first 16-byte boundary of the output on (the
-  // output itself is 16-byte aligned); single words before and after.
-  const uint32_t head = (uint32_t)min<uint64_t>((4u - (uint32_t)(first & 3u)) & 3u, words);
-  const uint64_t quads = (words - head) / 4;
-  if (threadIdx.x < head) trip[first + threadIdx.x] = s_trip[threadIdx.x];
-  uint4 *dst = reinterpret_cast<uint4 *>(trip + first + head);
-  for (uint64_t q = threadIdx.x; q < quads; q += 256) {
-    const uint32_t *src = s_trip + head + 4 * q;
-    dst[q] = make_uint4(src[0], src[1], src[2], src[3]);
-  }
-  for (uint64_t w = head + 4 * quads + threadIdx.x; w < words; w += 256)
-    trip[first + w] = s_trip[w];
 }
 
 // Emits into device triplets; returns the total number of qualifying cells.
