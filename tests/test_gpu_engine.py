@@ -150,23 +150,31 @@ def test_distance_pair_matches_oracle(rng):
             assert got == O.distance_dense(pa[0], pb[0], md)
 
 
-def test_stream_bands_concatenate_to_triangle(rng):
+# (2, 65535): no early-exit votes (the cap is out of reach of 200 sites);
+# the others vote, and their bands start at a cell offset inside the
+# triangle, stored through LDS: 128 x 32 tiles under a cap below 64,
+# 256 x 64 above.
+@pytest.mark.parametrize("out_bytes,md", [(2, 65535), (1, 3), (1, 100), (2, 150)])
+def test_stream_bands_concatenate_to_triangle(rng, out_bytes, md):
+    import ctypes
+
     n, L = 1500, 200
     seqs = random_strings(rng, n, L, "ACGT-N")
     packed = np.ascontiguousarray(O.pack(O.as_chars(seqs)))
-    want = O.all_pairs_dense(packed, 2, 65535)
+    want = O.all_pairs_dense(packed, out_bytes, md)
     got = np.zeros_like(want)
     seen = []
+    ctype = ctypes.c_uint16 if out_bytes == 2 else ctypes.c_uint8
 
     def cb(addr, first, count, r0, r1):
-        import ctypes
-        buf = (ctypes.c_uint16 * count).from_address(addr)
-        got[first:first + count] = np.frombuffer(buf, np.uint16)
+        buf = (ctype * count).from_address(addr)
+        got[first:first + count] = np.frombuffer(buf, want.dtype)
         assert first == r0 * (r0 - 1) // 2
         assert first + count == r1 * (r1 - 1) // 2
         seen.append((r0, r1))
 
-    capi.all_pairs_stream(packed, n, packed.shape[1], capi.HD_SEM_DENSE, 2, 65535, 100_000, cb)
+    capi.all_pairs_stream(packed, n, packed.shape[1], capi.HD_SEM_DENSE, out_bytes, md, 100_000,
+                          cb)
     assert len(seen) > 3
     assert seen[0][0] == 1 and seen[-1][1] == n
     np.testing.assert_array_equal(got, want)
