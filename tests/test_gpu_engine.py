@@ -270,3 +270,30 @@ def test_large_triangle_download_in_chunks(out_bytes, rng):
     got = engine_all_pairs(packed, capi.HD_SEM_DENSE, out_bytes, md)
     want = O.all_pairs_dense(packed, out_bytes, md)
     np.testing.assert_array_equal(got, want)
+
+
+def test_concurrent_host_calls_match_sequential(rng):
+    # Host threads calling the library at once (ctypes releases the GIL):
+    # the calls share the library streams and the pinned staging buffers
+    # (inputs and triangles over 64 MiB go through them).  Every result must
+    # equal the same call made alone; the small job is also checked against
+    # the oracle.
+    from concurrent.futures import ThreadPoolExecutor
+
+    table = np.frombuffer(b"ACGT-N", np.uint8)
+    jobs = []
+    for n, L, out_bytes, md in [(5000, 30000, 1, 255), (6000, 2000, 4, 2**32 - 1),
+                                (3000, 12000, 2, 65535), (700, 1000, 1, 3), (2500, 29903, 1, 2)]:
+        chars = table[rng.integers(0, table.size, size=(n, L), dtype=np.uint8)]
+        # a few near-duplicates so that the small caps are not all passed at once
+        chars[1::7] = chars[0::7][: chars[1::7].shape[0]]
+        jobs.append((O.pack(chars), out_bytes, md))
+    alone = [engine_all_pairs(p, capi.HD_SEM_DENSE, ob, md) for p, ob, md in jobs]
+    small = jobs[3]
+    np.testing.assert_array_equal(alone[3], O.all_pairs_dense(small[0], small[1], small[2]))
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        futures = [ex.submit(engine_all_pairs, p, capi.HD_SEM_DENSE, ob, md)
+                   for _ in range(2) for p, ob, md in jobs]
+        got = [f.result() for f in futures]
+    for k, g in enumerate(got):
+        np.testing.assert_array_equal(g, alone[k % len(jobs)], err_msg=f"job {k % len(jobs)}")
