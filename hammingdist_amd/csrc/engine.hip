@@ -139,8 +139,9 @@ int default_variant(const Geometry &g, bool early_exit, uint32_t max_dist) {
   if (votes && max_dist < kSmallCap) return 4;
   // The kGap codes' kernel runs 256 x 32 tiles at 6 waves per SIMD (77
   // VGPRs): at n = 100,000, L = 10,000 without votes 144 ms, against 159 ms
-  // for 256 x 64 tiles at 5 waves.
-  if (g.code.op == kGap2 || g.code.op == kGap3) return 2;
+  // for 256 x 64 tiles at 5 waves.  With votes, 128 x 32 tiles (cap 255 on
+  // the same data: 52.7 ms against 55.8 ms).
+  if (g.code.op == kGap2 || g.code.op == kGap3) return votes ? 4 : 2;
   // The dense 3-op code: 256 x 64 tiles.  128 x 64 tiles (8 waves per SIMD)
   // ran 0.7 % faster at n = 50,000, L = 29,903 but fetch twice the bytes
   // from HBM (240 GB against 124 GB per launch: each tile streams its

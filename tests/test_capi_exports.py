@@ -124,9 +124,9 @@ def test_default_tile_shape():
     # a cap out of reach of the row length turns the votes off
     assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 65535) == (4, 16)
     assert shape(1000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 255) == (2, 16)
-    # gap codes: 256 x 32 tiles, with and without votes
+    # gap codes: 256 x 32 tiles without votes, 128 x 32 with them
     assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, False, 65535) == (4, 8)
-    assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, True, 255) == (4, 8)
+    assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, True, 255) == (2, 8)
     assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, True, 3) == (2, 8)
     # the nibble code (dense data holding X) and the one-hot code: 128 x 64
     assert shape(30000, 5000, capi.HD_SEM_DENSE, acgt_gap_inv | (1 << 3), False, 65535) == \
