@@ -222,6 +222,17 @@ def test_sparse_emit_dev_on_resident_triangle(out_bytes, rng):
         if count > 1:
             with pytest.raises(capi.EngineError, match="capacity"):
                 capi.sparse_emit_dev(d_cells, n, out_bytes, thr, d_trip, count - 1, stream)
+        if count > 100:
+            # a capacity inside a workgroup's run: the first `cap` triplets
+            # are written and nothing past them
+            cap = count // 2 + 7
+            d_cap = torch.full((count * 3,), 0x7F7F7F7F, dtype=torch.int32, device="cuda")
+            with pytest.raises(capi.EngineError, match="capacity"):
+                capi.sparse_emit_dev(d_cells, n, out_bytes, thr, d_cap, cap, stream)
+            torch.cuda.synchronize()
+            got_cap = d_cap.cpu().numpy().view(np.uint32)
+            np.testing.assert_array_equal(got_cap[:cap * 3], trip.reshape(-1)[:cap * 3])
+            assert (got_cap[cap * 3:] == 0x7F7F7F7F).all()
 
 
 _long_rows = {}
