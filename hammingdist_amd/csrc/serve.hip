@@ -125,7 +125,10 @@ __global__ __launch_bounds__(256) void k_sq_count(const QueryDev *__restrict__ q
                                                   uint32_t *__restrict__ counts) {
   const QueryDev &Q = qs[block_query(qs, n_q)];
   const uint64_t k = (blockIdx.x - Q.block_begin) * kRowsPerBlock + threadIdx.x;
-  if (k < Q.m) atomicAdd(&counts[Q.rows[k]], 1u);
+  if (k < Q.m) {
+    atomicAdd(&counts[Q.rows[k]], 1u);
+    Q.out[k] = 0u; // the shared scan adds its two half-row sums into it
+  }
 }
 
 // starts[row] (exclusive prefix of the counts) advances past each entry
@@ -210,34 +213,44 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *__restrict__ v, uint
     v[i] = 0u;
 }
 
-// Persistent wavefronts over the database rows; ends[r] is the end of row
-// r's entries (its start is ends[r - 1]).  The wavefront holds row r in
-// VGPRs (non-temporal loads: the row is read once) and streams each entry's
-// query against it (cached loads: the group's queries are read again for
-// every row).  Per (query, row) pair this took 2.5 ns on the serving
-// workload against 3.7 ns for scanning each query's rows on its own; staging
-// the row in LDS for a workgroup's four wavefronts was slower (the row's
-// load and its entries serialise within the workgroup).
+// Persistent teams of kSqParts wavefronts over the database rows; ends[r]
+// is the end of row r's entries (its start is ends[r - 1]).  Each wavefront
+// of a team holds one part of row r in VGPRs (non-temporal loads: the row is
+// read once), streams that part of each entry's query against it (cached
+// loads: the group's queries are read again for every row) and adds its
+// count into the entry's output cell, which the count pass zeroed.  Per
+// (query, row) pair, 24 queries x 300,000 rows of 14,952 B: whole rows 1.48
+// ns (228 VGPRs, 2 wavefronts per SIMD), halves 1.33 ns (98 VGPRs, 4 per
+// SIMD), quarters 1.44 ns; scanning each query's rows on its own takes 2.5
+// ns.  Staging the row in LDS for a workgroup's four wavefronts was slower
+// (the row's load and its entries serialise within the workgroup).
+constexpr uint32_t kSqParts = 2; // wavefronts per database row in the shared scan
+
 template <int SEM, int T>
 __global__ __launch_bounds__(256) void k_sq_scan(const QueryDev *__restrict__ qs,
                                                  const uint8_t *__restrict__ db, uint64_t bpr,
                                                  uint64_t n_db, const uint32_t *__restrict__ ends,
                                                  const SharedEntry *__restrict__ ent) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
-  const uint64_t w0 = (uint64_t)blockIdx.x * (blockDim.x / 64) +
-                      __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  constexpr uint32_t kTeam = 64 * kSqParts;
+  const uint64_t teams = (uint64_t)gridDim.x * (blockDim.x / kTeam);
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const uint64_t t0 = (uint64_t)blockIdx.x * (blockDim.x / kTeam) + wave / kSqParts;
   const uint32_t units = (uint32_t)(bpr / 8);
-  for (uint64_t r = w0; r < n_db; r += waves) {
+  const uint32_t part_units = (units + kSqParts - 1) / kSqParts;
+  const uint32_t part = wave % kSqParts;
+  const uint32_t first = min(units, part * part_units);           // this part's first unit
+  const uint32_t count = min(units, first + part_units) - first;
+  for (uint64_t r = t0; r < n_db; r += teams) {
     const uint32_t e1 = ends[r], e0 = r ? ends[r - 1] : 0u;
     if (e0 == e1) continue;
     RegQuery<SEM, T> row;
-    row.template load<true>(reinterpret_cast<const uint2 *>(db + r * bpr), units, lane);
+    row.template load<true>(reinterpret_cast<const uint2 *>(db + r * bpr) + first, count, lane);
     for (uint32_t e = e0; e < e1; ++e) {
       const SharedEntry E = ent[e];
       const QueryDev &Q = qs[E.q];
-      const uint32_t d = row.template distance<false>(Q.query, units, lane);
-      if (lane == 0) Q.out[E.pos] = d;
+      const uint32_t d = row.template distance<false>(Q.query + first, count, lane);
+      if (lane == 0) atomicAdd(Q.out + E.pos, d);
     }
   }
 }
@@ -259,7 +272,7 @@ static const void *shared_scan_kernel(int semantics, uint64_t bpr) {
 #define HD_SQ(T)                                                                          \
   return semantics == HD_SEM_DENSE ? reinterpret_cast<const void *>(&k_sq_scan<HD_SEM_DENSE, T>) \
                                    : reinterpret_cast<const void *>(&k_sq_scan<HD_SEM_CHAR_X, T>);
-  switch (reg_query_t(bpr / 8)) {
+  switch (reg_query_t((bpr / 8 + kSqParts - 1) / kSqParts)) { // row parts
   case 4: HD_SQ(4)
   case 8: HD_SQ(8)
   case 16: HD_SQ(16)
