@@ -358,20 +358,37 @@ void launch_encode(const Geometry &g, const uint8_t *d_packed, uint32_t *d_plane
 __global__ __launch_bounds__(256) void k_nibble_mask(const uint2 *__restrict__ data,
                                                      uint64_t n_units,
                                                      uint32_t *__restrict__ mask) {
+  // Four independent 8-byte loads per thread and step keep enough bytes in
+  // flight to stream the rows (one load per step left the kernel at about
+  // half the HBM rate).
   uint32_t m = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n_units;
-       u += (uint64_t)gridDim.x * blockDim.x) {
-    const uint2 v = data[u];
+       u += 4 * stride) {
+    uint2 v[4];
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      m |= 1u << ((v.x >> (4 * s)) & 15u);
-      m |= 1u << ((v.y >> (4 * s)) & 15u);
+    for (int k = 0; k < 4; ++k) {
+      // a unit past the end repeats unit u, which adds no new code
+      const uint64_t x = u + k * stride;
+      v[k] = data[x < n_units ? x : u];
     }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        m |= 1u << ((v[k].x >> (4 * s)) & 15u);
+        m |= 1u << ((v[k].y >> (4 * s)) & 15u);
+      }
   }
-  // OR across the wavefront, then one atomic per wavefront.
+  // OR across the wavefront, then across the workgroup, then one atomic per
+  // workgroup (atomics on one address serialise: one per wavefront of a
+  // 4096-block grid took more time than reading the rows).
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off);
-  if ((threadIdx.x & 63) == 0) atomicOr(mask, m);
+  __shared__ uint32_t s_m[4];
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicOr(mask, s_m[0] | s_m[1] | s_m[2] | s_m[3]);
 }
 
 void launch_nibble_mask(const uint8_t *d_packed, uint64_t bytes, uint32_t *d_mask,
@@ -379,7 +396,7 @@ void launch_nibble_mask(const uint8_t *d_packed, uint64_t bytes, uint32_t *d_mas
   const uint64_t units = bytes / 8;
   HD_HIP(hipMemsetAsync(d_mask, 0, sizeof(uint32_t), s));
   if (!units) return;
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>((units + 255) / 256, 4096);
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((units + 255) / 256, 1024);
   hipLaunchKernelGGL(k_nibble_mask, dim3(blocks), dim3(256), 0, s,
                      reinterpret_cast<const uint2 *>(d_packed), units, d_mask);
   HD_HIP(hipGetLastError());
