@@ -116,11 +116,11 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
 // ---------------------------------------------------------------------------
 // Rows per lane, columns per wave, waves per workgroup.
 static const Variant kVariants[] = {
-    {4, 16, 4}, // 256 x 64 tiles: the large-n shape for 4 planes
+    {4, 16, 4}, // 256 x 64 tiles: the dense 3-op code at large n
     {8, 8, 4},  // 512 x 32
-    {4, 8, 4},  // 256 x 32: 5 planes (fewer SGPRs per column block)
-    {2, 16, 4}, // 128 x 64
-    {2, 8, 4},  // 128 x 32: small datasets
+    {4, 8, 4},  // 256 x 32: the gap codes (fewer SGPRs per column block)
+    {2, 16, 4}, // 128 x 64: the pointer-addressed codes, n < 2048
+    {2, 8, 4},  // 128 x 32: small datasets, early exit under small caps
 };
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
