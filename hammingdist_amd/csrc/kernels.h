@@ -128,6 +128,13 @@ __host__ __device__ constexpr bool has_corr(int P) {
   return P == kDense3 || P == kGap2 || P == kGap3;
 }
 
+// A wave-uniform word copied into a VGPR (one v_mov_b32).
+__device__ __forceinline__ uint32_t to_vgpr(uint32_t x) {
+  uint32_t v;
+  __asm__("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+  return v;
+}
+
 // Tile t -> (row block b, column block cb) of a triangle tiled with M column
 // blocks per row block: row block b holds tiles [M b(b+1)/2, M (b+1)(b+2)/2).
 __device__ __forceinline__ void tile_coords(uint64_t t, uint32_t M, uint32_t &b,
@@ -241,6 +248,11 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   constexpr int PA = GAP ? GB : PS;      // row planes used by the main loop
   constexpr int PB = D3 ? 3 : PA;        // column planes used
   constexpr int B0 = D3 ? 1 : 0;         // first column plane used
+  // Column words through VGPRs (one v_mov_b32 per column plane and step):
+  // the dense 3-op code at n = 50,000, L = 29,903 went from 111.0 to 99.5 ms;
+  // the gap-3 code at n = 100,000, L = 10,000 from 144.7 to 222.9 ms (one
+  // wave less per SIMD), so it keeps the SGPR operands.
+  constexpr bool VCOL = D3;
   // Offset codes: accumulators carry OFF + popcounts + the row and column
   // counts added so far.  Rows have fewer than 2^23 sites (kBufWords; the
   // codes are only chosen then), so every partial sum of a pair stays
@@ -340,41 +352,48 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
 #pragma unroll
         for (int c = 0; c < C; ++c) bw[c][p] = pb[(B0 + p) * kWave + c];
 #pragma unroll
-      for (int q = 0; q < R; ++q)
+      for (int c = 0; c < C; ++c) {
+        // VCOL: the column's words are copied into VGPRs first; v_bitop3
+        // with three VGPR operands issues up to twice per CU and clock on
+        // gfx950, with an SGPR operand once.
+        uint32_t b[PB];
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
+        for (int p = 0; p < PB; ++p) b[p] = VCOL ? to_vgpr(bw[c][p]) : bw[c][p];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
           // v_bitop3 truth tables are over src0 = 0xF0, src1 = 0xCC,
           // src2 = 0xAA.
           uint32_t s;
           if constexpr (D3) {
-            s = __builtin_amdgcn_bitop3_b32(a[q][1], bw[c][0], a[q][0], 0xBE); // z|(x^y)
-            s = __builtin_amdgcn_bitop3_b32(a[q][2], bw[c][1], s, 0x41);
-            s = __builtin_amdgcn_bitop3_b32(a[q][3], bw[c][2], s, 0x71);
+            s = __builtin_amdgcn_bitop3_b32(a[q][1], b[0], a[q][0], 0xBE); // z|(x^y)
+            s = __builtin_amdgcn_bitop3_b32(a[q][2], b[1], s, 0x41);
+            s = __builtin_amdgcn_bitop3_b32(a[q][3], b[2], s, 0x71);
           } else if constexpr (GAP) {
             // codes differ: XOR, then 0xBE = (x ^ y) | z per further bit
-            s = a[q][0] ^ bw[c][0];
+            s = a[q][0] ^ b[0];
 #pragma unroll
             for (int p = 1; p < GB; ++p)
-              s = __builtin_amdgcn_bitop3_b32(a[q][p], bw[c][p], s, 0xBE);
+              s = __builtin_amdgcn_bitop3_b32(a[q][p], b[p], s, 0xBE);
           } else if constexpr (CLS) {
             // code bits differ (0xBE = (x^y)|z), then z & ~x & ~y (0x02)
             // with the two '-' planes
-            s = a[q][0] ^ bw[c][0];
+            s = a[q][0] ^ b[0];
 #pragma unroll
             for (int p = 1; p < PS - 1; ++p)
-              s = __builtin_amdgcn_bitop3_b32(a[q][p], bw[c][p], s, 0xBE);
-            s = __builtin_amdgcn_bitop3_b32(a[q][PS - 1], bw[c][PS - 1], s, 0x02);
+              s = __builtin_amdgcn_bitop3_b32(a[q][p], b[p], s, 0xBE);
+            s = __builtin_amdgcn_bitop3_b32(a[q][PS - 1], b[PS - 1], s, 0x02);
           } else {
             // s = a0&b0; s = (ap&bp)|s ...; s = ~((a_last&b_last)|s):
             // 0xEA = (x&y)|z, 0x15 = its NOT.
-            s = a[q][0] & bw[c][0];
+            s = a[q][0] & b[0];
 #pragma unroll
             for (int p = 1; p < P - 1; ++p)
-              s = __builtin_amdgcn_bitop3_b32(a[q][p], bw[c][p], s, 0xEA);
-            s = __builtin_amdgcn_bitop3_b32(a[q][P - 1], bw[c][P - 1], s, 0x15);
+              s = __builtin_amdgcn_bitop3_b32(a[q][p], b[p], s, 0xEA);
+            s = __builtin_amdgcn_bitop3_b32(a[q][P - 1], b[P - 1], s, 0x15);
           }
           acc[q][c] += __builtin_popcount(s);
         }
+      }
       pa += PS * kWave;
       pb += PS * kWave;
     }

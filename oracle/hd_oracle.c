@@ -164,6 +164,29 @@ static int sparse_merge(const uint64_t *ia_, const uint8_t *ca, uint64_t na,
   return r < max_dist ? r : max_dist;
 }
 
+/* One cell of the sparse route: to_sparse_data of both sequences against the
+ * dataset's consensus `cons` (src/hamming_impl.cc:170-188), then
+ * distance_sparse (src/hamming_impl.cc:93-129).  For sampled cells of
+ * datasets too large for oracle_all_pairs_sparse; cons comes from
+ * oracle_consensus over the whole dataset.                                   */
+int oracle_distance_sparse_pair(const char *a, const char *b, const char *cons, uint64_t L,
+                                int include_x, uint32_t max_dist) {
+  uint8_t lut[256];
+  oracle_lookup(include_x, lut);
+  uint64_t *ia = (uint64_t *)malloc((L ? L : 1) * sizeof(uint64_t));
+  uint64_t *ib = (uint64_t *)malloc((L ? L : 1) * sizeof(uint64_t));
+  uint8_t *ca = (uint8_t *)malloc(L ? L : 1);
+  uint8_t *cb = (uint8_t *)malloc(L ? L : 1);
+  uint64_t na = sparse_list(a, cons, L, lut, ia, ca);
+  uint64_t nb = sparse_list(b, cons, L, lut, ib, cb);
+  int d = sparse_merge(ia, ca, na, ib, cb, nb, md_int(max_dist));
+  free(ia);
+  free(ib);
+  free(ca);
+  free(cb);
+  return d;
+}
+
 /* safe_int_cast<T>(max_distance), include/hamming/hamming_impl.hh:37-45 */
 uint32_t oracle_clamp_max_distance(int64_t max_distance, int out_bytes) {
   uint64_t tmax = out_bytes == 1 ? 0xFFu : out_bytes == 2 ? 0xFFFFu : 0xFFFFFFFFu;
@@ -304,7 +327,7 @@ dist_avx512(const uint8_t *a, const uint8_t *b, uint64_t bytes, int max_dist) {
 int oracle_have_avx512(void) {
 #if defined(__x86_64__)
   __builtin_cpu_init();
-  return __builtin_cpu_supports("avx512bw");
+  return __builtin_cpu_supports("avx512bw") != 0;
 #else
   return 0;
 #endif

@@ -24,6 +24,7 @@ _SIGS = {
     "oracle_use_sparse": (C.c_int, [vp, u64, u64, C.c_int]),
     "oracle_all_pairs_sparse": (None, [vp, u64, u64, C.c_int, C.c_int, C.c_uint32, vp]),
     "oracle_all_pairs_dense": (None, [vp, u64, u64, C.c_int, C.c_uint32, vp]),
+    "oracle_distance_sparse_pair": (C.c_int, [vp, vp, vp, u64, C.c_int, C.c_uint32]),
     "oracle_distances": (C.c_int, [vp, u64, u64, C.c_int, C.c_int, C.c_int, i64, vp]),
     "oracle_have_avx512": (C.c_int, []),
     "oracle_all_pairs_dense_simd": (C.c_int, [vp, u64, u64, u64, u64, C.c_int, C.c_uint32,
@@ -148,3 +149,53 @@ def tri_index(n: int):
     i, j = np.tril_indices(n, -1)
     order = np.lexsort((j, i))
     return i[order], j[order]
+
+
+def distance_sparse_pair(a: np.ndarray, b: np.ndarray, cons: bytes, include_x: bool,
+                         max_dist: int = 2**32 - 1) -> int:
+    """One cell of the sparse route: rows a, b (uint8 chars) against the
+    dataset consensus `cons` (from consensus() over the whole dataset)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return int(lib().oracle_distance_sparse_pair(a.ctypes.data, b.ctypes.data, cons, a.size,
+                                                 int(include_x), max_dist))
+
+
+def all_pairs_dense_simd(packed: np.ndarray, row0: int, row1: int, out_bytes: int,
+                         max_dist: int, threads: int = 0) -> np.ndarray:
+    """Rows [row0, row1) of the triangle through the oracle's restatement of
+    the reference's AVX-512 pair kernel (scalar kernel where the CPU lacks
+    AVX-512BW)."""
+    n, bps = packed.shape
+    first = row0 * (row0 - 1) // 2 if row0 else 0
+    out = np.zeros(row1 * (row1 - 1) // 2 - first if row1 else 0, DTYPE[out_bytes])
+    p = np.ascontiguousarray(packed)
+    lib().oracle_all_pairs_dense_simd(p.ctypes.data, n, bps, row0, row1, out_bytes, max_dist,
+                                      out.ctypes.data, threads)
+    return out
+
+
+def sample_cells(rng: np.random.Generator, n: int, k: int) -> np.ndarray:
+    """k linear triangle indices spread over the whole triangle of n rows:
+    uniform draws plus the first cells, the whole last row's ends, and the
+    cells around 2^31 and 2^32 where the triangle reaches them."""
+    cells = n * (n - 1) // 2
+    picks = [rng.integers(0, cells, size=k, dtype=np.uint64)]
+    fixed = [0, 1, 2, cells - 1, cells - 2, cells - (n - 1), cells - (n - 1) + 1]
+    for edge in (2**31, 2**32):
+        if cells > edge + 2:
+            fixed += [edge - 2, edge - 1, edge, edge + 1]
+    picks.append(np.array([x for x in fixed if 0 <= x < cells], np.uint64))
+    return np.unique(np.concatenate(picks))
+
+
+def cell_rows(x: np.ndarray):
+    """(i, j) of linear triangle indices x (i > j, cell at i(i-1)/2 + j)."""
+    x = np.asarray(x, np.uint64)
+    i = np.floor(np.sqrt(2.0 * x.astype(np.float64) + 0.25) + 0.5).astype(np.uint64)
+    # exact integer correction of the floating-point row
+    for _ in range(2):
+        i = np.where(i * (i - 1) // 2 > x, i - 1, i)
+        i = np.where((i + 1) * i // 2 <= x, i + 1, i)
+    j = x - i * (i - 1) // 2
+    return i.astype(np.int64), j.astype(np.int64)

@@ -117,6 +117,77 @@ def test_dense_equals_reference_simd_kernels():
                     assert got == want, (L, md, which)
 
 
+SIMD_LENGTHS = [1, 63, 64, 65, 127, 128, 129, 8191, 29903]
+SIMD_CAPS = [0, 1, 2, 16, 254, 255, 256, 65535]
+
+
+@pytest.mark.parametrize("L", SIMD_LENGTHS)
+def test_simd_restatement_equals_scalar_and_reference(L):
+    # The oracle's AVX-512 restatement (distance_avx512's u8 lane counters and
+    # n_inner early-exit cadence under distances<T>'s row loop) against the
+    # scalar distance_cpp restatement on every cell, as the reference's own
+    # src/distance_avx512_t.cc:6-73 pins its SIMD kernel against distance_cpp;
+    # and against the reference's compiled AVX-512 kernel where oracle/_ref is
+    # built.  Rows hold long runs of disagreement (distances up to L, past the
+    # u8 lane range of one inner block) and identical pairs (distance 0).
+    rng = np.random.default_rng(7000 + L)
+    n = 23
+    base = rng.choice(list("ACGT"), size=L)
+    seqs = []
+    for r in range(n):
+        s = base.copy()
+        k = int(rng.integers(0, L + 1)) if r % 4 else (0 if r % 8 == 0 else L)
+        pos = rng.choice(L, size=k, replace=False)
+        s[pos] = rng.choice(list("ACGT-NX"), size=k)
+        seqs.append("".join(s))
+    seqs.append(seqs[5])  # an identical pair
+    p = O.pack(O.as_chars(seqs))
+    n = len(seqs)
+    ref = O.ref()
+    for md in SIMD_CAPS:
+        for ob in (1, 2, 4):
+            tmax = (1 << (8 * ob)) - 1
+            if md > tmax:
+                continue
+            want = O.all_pairs_dense(p, ob, md)
+            got = O.all_pairs_dense_simd(p, 0, n, ob, md, threads=3)
+            np.testing.assert_array_equal(got, want, err_msg=f"L={L} md={md} ob={ob}")
+            # bands starting past row 0: cells relative to the band's first cell
+            for r0, r1 in [(1, 2), (2, 9), (9, n), (n - 1, n)]:
+                first = r0 * (r0 - 1) // 2
+                band = O.all_pairs_dense_simd(p, r0, r1, ob, md, threads=2)
+                np.testing.assert_array_equal(band, want[first:r1 * (r1 - 1) // 2],
+                                              err_msg=f"L={L} md={md} band {r0}:{r1}")
+            if ref is not None and ob == 4:
+                out = np.zeros_like(want)
+                ref.ref_all_pairs(3, p.ctypes.data, n, p.shape[1], 0, n, 4, md,
+                                  out.ctypes.data, 2, None)
+                np.testing.assert_array_equal(out, want, err_msg=f"ref L={L} md={md}")
+
+
+@pytest.mark.parametrize("alphabet", ["ACGT-NX", "AAAAAACGT", "ACGTX", "A-"])
+def test_sparse_pair_equals_sparse_all_pairs(alphabet, rng):
+    # The one-cell sparse oracle (lists against the whole dataset's consensus)
+    # gives every cell of the all-pairs sparse route, under every cap.
+    for n, L in [(2, 1), (9, 33), (40, 257)]:
+        chars = O.as_chars(random_strings(rng, n, L, alphabet))
+        for include_x in (False, True):
+            cons = O.consensus(chars, include_x)
+            I, J = O.tri_index(n)
+            for md in (0, 1, 3, 65535):
+                want = O.all_pairs_sparse(chars, include_x, 2, md)
+                got = [O.distance_sparse_pair(chars[i], chars[j], cons, include_x, md)
+                       for i, j in zip(I, J)]
+                np.testing.assert_array_equal(np.array(got, np.uint16), want)
+
+
+def test_simd_restatement_runs_avx512_here():
+    # the restatement above is only pinned where it runs its AVX-512 path
+    if not O.lib().oracle_have_avx512():
+        pytest.skip("this CPU lacks AVX-512BW: the restatement falls back to the scalar kernel")
+    assert O.lib().oracle_have_avx512() == 1
+
+
 def test_dense_brute_force_nibble_rule():
     # every ordered pair of chars over a mixed alphabet: dense counts a site
     # exactly when the lookup codes share no bit
