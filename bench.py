@@ -32,20 +32,43 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "site-comparisons/s all-pairs n=50k L=29903; batched vs plain jobs/s; p99 ms"
-# Nominal INT32 VALU rate from MI355X's published CU count and peak clock:
-# 256 CUs x 64 lanes x 2.4 GHz lane-ops/s.  The issue rate measured by
-# hd_valu_probe on the same instruction mix is reported beside it.
-PEAK_INT32_OPS = 256 * 64 * 2.4e9
+# VALU issue model of gfx950 (MI355X: 256 CUs, 2.4 GHz peak engine clock),
+# from hd_valu_probe's split instruction chains (profiles/valu_issue_rates.md):
+#   - "dual-rate" instructions issue up to twice per CU per clock: v_bitop3_b32,
+#     v_xor_b32, v_add_u32 and v_fma_f32 with VGPR operands only.  Twice per
+#     clock is also what MI355X's published FP32 vector rate implies
+#     (157.3 TFLOP/s = 256 CUs x 2.4 GHz x 128 lanes x 2 flops per FMA).
+#   - "single-rate" instructions issue once per CU per clock: v_bcnt_u32_b32,
+#     and v_bitop3_b32 reading an SGPR operand.
+# The peak of a loop is then ops / (dual / 2 + single) wave64 instructions per
+# CU per clock, times 64 lanes x 256 CUs x 2.4 GHz.  NOMINAL_VALU (one
+# instruction per CU per clock) is reported beside it.
+CUS, LANES, CLOCK = 256, 64, 2.4e9
+NOMINAL_VALU = CUS * LANES * CLOCK
 PEAK_HBM = 8.0e12
-# VALU ops per pair per 32-site word, by stored plane count: P + 1 for the
-# one-hot nibble codes
-OPS_PER_WORD = {3: 4, 4: 5, 5: 6}
-OPS_PER_WORD_DENSE3 = 4  # the dense 3-op code: three v_bitop3 + popcount-accumulate
-# The sparse codes with '-' as a code: B code bits (B = P - 1 stored planes
-# besides the '-' plane), B - 1 v_bitop3 after one XOR, plus the
-# popcount-accumulate: P ops.  (The 2 |g_i & g_j| pass over words holding a
-# '-' is not counted.)
-OPS_PER_WORD_GAP = {3: 3, 4: 4}
+
+
+def valu_peak(dual: int, single: int) -> float:
+    """Lane-ops/s bound of a loop of `dual` dual-rate and `single`
+    single-rate VALU instructions per unit of work."""
+    return (dual + single) / (dual / 2 + single) * NOMINAL_VALU
+
+
+# Algorithmic VALU ops per pair and 32-site word, as (dual-rate, single-rate):
+# every code is a chain of bitwise ops (v_bitop3 / v_xor / v_and, dual-rate
+# with VGPR operands) and one popcount-accumulate (v_bcnt, single-rate).
+#   the dense 3-op code: three v_bitop3 + v_bcnt
+#   the nibble code with P planes: P bitwise ops + v_bcnt
+#   the sparse gap codes with P stored planes (P - 1 code bits): one XOR,
+#   P - 2 v_bitop3 + v_bcnt (the 2 |g_i & g_j| pass over words holding a '-'
+#   is not counted)
+MIX_DENSE3 = (3, 1)
+MIX_NIBBLE = {3: (3, 1), 4: (4, 1), 5: (5, 1)}
+MIX_GAP = {3: (2, 1), 4: (3, 1)}
+# The reference's own encoding: 4-bit nibbles, 8 sites per 32-bit word, and
+# per word pair AND, two shift-ORs, a mask and a popcount-accumulate
+# (src/distance_cuda.cu:57-65): 5 ops per 8 sites.
+NIBBLE_OPS_PER_SITE = 5 / 8
 
 
 def log(*a):
@@ -268,15 +291,18 @@ def run_sparse_emit(torch, capi, np, d_cells, host_head, n, md, sh):
 
 
 def valu_ceiling():
-    """Lane-ops/s of the VALU issue-rate probe (hammingdist_amd/hd_valu_probe:
-    v_bitop3 + popcount chains, no memory), or None when it is not built."""
+    """Issue rates of the VALU probe (hammingdist_amd/hd_valu_probe: long
+    chains of one instruction mix each, no memory), {mode: lane-ops/s,
+    shader clock, instructions per CU per clock}; None when not built."""
     exe = os.path.join(ROOT, "hammingdist_amd", "hd_valu_probe")
     if not os.path.exists(exe):
         return None
     try:
-        out = subprocess.run([exe], capture_output=True, text=True, timeout=120, check=True)
-        return json.loads(out.stdout.strip().splitlines()[-1])
-    except (subprocess.SubprocessError, ValueError, IndexError) as e:
+        out = subprocess.run([exe, "all"], capture_output=True, text=True, timeout=120,
+                             check=True)
+        rows = [json.loads(line) for line in out.stdout.strip().splitlines()]
+        return {r["mode"]: r for r in rows}
+    except (subprocess.SubprocessError, ValueError, KeyError) as e:
         log(f"valu probe failed: {e}")
         return None
 
@@ -348,8 +374,11 @@ def run_large(args, torch, capi, workloads, np):
     defer_check("large all-pairs cells (64)", check_large)
 
     site_cmp = cells * args.L
-    ops = cells * W * (OPS_PER_WORD_DENSE3 if dense3 else OPS_PER_WORD[P])
+    dual, single = MIX_DENSE3 if dense3 else MIX_NIBBLE[P]
+    ops = cells * W * (dual + single)
     achieved = ops / (kern_ms / 1e3)
+    peak = valu_peak(dual, single)
+    nibble = NIBBLE_OPS_PER_SITE * site_cmp / (kern_ms / 1e3)
     variant = (args.variant if args.variant >= 0
                else capi.default_variant(n, bps, sem, mask, False, md))
     shape = capi.variants()[variant]
@@ -371,13 +400,23 @@ def run_large(args, torch, capi, workloads, np):
         "roofline": {
             "bound": "valu",
             "achieved": achieved / 1e12,
-            "peak": PEAK_INT32_OPS / 1e12,
+            "peak": peak / 1e12,
             "unit": "TOP/s",
-            "frac": achieved / PEAK_INT32_OPS,
+            "frac": achieved / peak,
             "traffic": measured_traffic(n, args.L, md, R, Cw),
             "kernel": kernel_name,
             "kernel_ms": kern_ms,
             "ops_per_launch": ops,
+            "ops_per_pair_word": {"dual_rate": dual, "single_rate": single},
+            "issue_model": "dual-rate VALU ops 2 per CU per clock, v_bcnt 1; "
+                           "256 CUs x 64 lanes x 2.4 GHz",
+            "nominal_peak": NOMINAL_VALU / 1e12,
+            "frac_of_nominal": achieved / NOMINAL_VALU,
+            # the same kernel time priced in the reference's nibble encoding
+            # (5 ops per 8 sites): work this kernel does not issue
+            "nibble_equivalent": {"ops_per_site": NIBBLE_OPS_PER_SITE,
+                                  "achieved": nibble / 1e12,
+                                  "frac_of_peak": nibble / peak},
         },
         "wall_s_timed": wall,
     }
@@ -459,10 +498,16 @@ def run_wide(args, torch, capi, workloads, np):
         assert np.array_equal(host, want), "wide: triangle differs from the oracle"
 
     defer_check("wide triangle, first 400 rows", check_wide)
-    ops = cells * W * OPS_PER_WORD_GAP[P]
+    dual, single = MIX_GAP[P]
+    ops = cells * W * (dual + single)
+    peak = valu_peak(dual, single)
     res = {"n": n, "L": spec.L, "cells": cells, "semantics": "sparse_x", "planes": P,
            "out": "uint16", "ms_per_step": ms, "site_comparisons_per_s": cells * spec.L / (ms / 1e3),
-           "valu_frac": ops / (ms / 1e3) / PEAK_INT32_OPS,
+           "roofline": {"bound": "valu", "achieved": ops / (ms / 1e3) / 1e12,
+                        "peak": peak / 1e12, "unit": "TOP/s",
+                        "frac": ops / (ms / 1e3) / peak,
+                        "frac_of_nominal": ops / (ms / 1e3) / NOMINAL_VALU,
+                        "note": "step time (encode + all-pairs)"},
            "triangle_GB": cells * 2 / 1e9}
     log(f"wide: {ms:.1f} ms/step, P={P}")
     # reference distances of a fresh mutant against all rows (CHAR_X)
@@ -840,14 +885,20 @@ def main() -> int:
 
     ceiling = valu_ceiling()
     if ceiling:
-        log(f"valu probe: {ceiling['lane_ops_per_s'] / 1e12:.2f} T lane-ops/s at "
-            f"{ceiling['shader_clock_ghz']:.3f} GHz")
+        log("valu probe: " + ", ".join(
+            f"{m} {r['valu_instr_per_cu_per_clock']:.2f}/CU/clk" for m, r in ceiling.items()))
     large, packed = run_large(args, torch, capi, workloads, np)
     if ceiling:
         rl = large["roofline"]
-        rl["measured_ceiling"] = ceiling["lane_ops_per_s"] / 1e12
-        rl["frac_of_measured_ceiling"] = rl["achieved"] / rl["measured_ceiling"]
-        rl["ceiling_shader_clock_ghz"] = ceiling["shader_clock_ghz"]
+        # the probe chain of the headline loop's form: per column word, three
+        # VGPR copies, then v_bitop3 x 3 + v_bcnt per row
+        form = ceiling.get("bcast")
+        if form:
+            rl["measured_ceiling"] = form["lane_ops_per_s"] / 1e12
+            rl["frac_of_measured_ceiling"] = rl["achieved"] / rl["measured_ceiling"]
+            rl["ceiling_shader_clock_ghz"] = form["shader_clock_ghz"]
+        rl["probe_instr_per_cu_per_clock"] = {
+            m: round(r["valu_instr_per_cu_per_clock"], 4) for m, r in ceiling.items()}
     result = {
         "metric": METRIC,
         "value": large["value"],

@@ -20,7 +20,7 @@ from ._runtime import ensure_single_runtime as _ensure_single_runtime
 _ensure_single_runtime()
 
 try:
-    from . import _hammingdist as _native
+    from . import hammingdist as _native
 except ImportError as exc:  # pragma: no cover - exercised only when unbuilt
     raise ImportError(
         "hammingdist_amd native libraries are not built "

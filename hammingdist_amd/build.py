@@ -2,7 +2,8 @@
 
   libhd_engine.so      HIP engine (csrc/*.hip) for gfx950, C ABI of
                        include/hammingdist_amd.h
-  _hammingdist*.so     C++ host layer + pybind11 module (csrc/host/*.cc),
+  hammingdist*.so      C++ host layer + pybind11 module `hammingdist`
+                       (csrc/host/*.cc),
                        linked against libhd_engine.so
 
 Run ``python -m hammingdist_amd.build`` (or ``__graft_entry__.build()``).
@@ -92,7 +93,7 @@ def build_probe() -> str:
 
 def _module_name() -> str:
     suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
-    return "_hammingdist" + suffix
+    return "hammingdist" + suffix
 
 
 def build_host() -> str:
@@ -115,6 +116,10 @@ def build_host() -> str:
     _run(["g++", "-shared", "-pthread", "-o", tmp, *objs, f"-L{HERE}", "-lhd_engine",
           "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, out)
+    # the module's former name, from older builds of this tree
+    for name in os.listdir(HERE):
+        if name.startswith("_hammingdist") and name.endswith(".so"):
+            os.remove(os.path.join(HERE, name))
     return out
 
 

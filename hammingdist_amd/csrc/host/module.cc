@@ -75,7 +75,10 @@ hdhost::SynthSpec spec_from(std::uint64_t n, std::uint64_t L, std::uint64_t seed
 
 } // namespace
 
-PYBIND11_MODULE(_hammingdist, m) {
+// The reference's module name (python/hammingdist.cc:28), so that callers
+// written against it `import hammingdist` unchanged (the top-level
+// hammingdist package re-exports this module).
+PYBIND11_MODULE(hammingdist, m) {
   using namespace hdhost;
   m.doc() = "Small tool to calculate Hamming distances between gene sequences "
             "(MI355X HIP engine)";
