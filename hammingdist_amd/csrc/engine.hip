@@ -545,12 +545,11 @@ static void launch_refdist(const uint8_t *d_query, const uint8_t *d_db,
 // 256-thread block kEmitCells = 4096 cells.  Pass 1 counts the qualifying
 // cells per block; a two-level scan on the device turns the counts into
 // block offsets (per 4096-block chunk, then across chunks); pass 2 re-reads
-// the cells, scans the per-thread hit counts inside the block and writes
-// each thread's hits in cell order.  Only the total goes to the host.
+// the cells and writes each block's hits in cell order (k_emit_write).  Only
+// the total goes to the host.
 static constexpr uint32_t kEmitPerThread = 16;
 static constexpr uint32_t kEmitCells = 256 * kEmitPerThread;
 static constexpr uint32_t kScanChunk = 256 * 16; // block counts per scan block
-static constexpr uint32_t kEmitRounds = 2;        // LDS turns of k_emit_write
 
 // The 16 cells x0 .. x0+15 as uint32 values; returns how many of them
 // exist (cells past n_cells read as 0 and must be ignored).  vec: x0 is
@@ -666,6 +665,19 @@ __device__ __forceinline__ uint64_t row_of_cell(uint64_t x) {
   return i;
 }
 
+// Pass 2.  Wave w of a block owns the block's cells [1024 w, 1024 (w + 1)):
+// the 16-byte loads put them into LDS, then in step e (0..15) lane l takes
+// cell 64 e + l, so the hits of one step are consecutive cells and their
+// triplets land at consecutive positions (a ballot and mbcnt give each
+// lane's position; triplet words 3 apart spread over all LDS banks).  Each
+// wave stages the triplets of 8 steps at a time in its own LDS (shifted so
+// that 16-byte quads of the output are 16-byte aligned in LDS) and stores
+// them with consecutive lanes on consecutive quads; only the wave totals
+// need a block barrier.
+static constexpr uint32_t kEmitWaveCells = 1024;
+static constexpr uint32_t kEmitHalf = 8;                        // steps per staging turn
+static constexpr uint32_t kEmitStageWords = 3 * 64 * kEmitHalf + 4; // per wave
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_emit_write(const T *__restrict__ cells,
                                                     uint64_t n_cells, uint32_t thr, int vec,
@@ -673,105 +685,132 @@ __global__ __launch_bounds__(256) void k_emit_write(const T *__restrict__ cells,
                                                     const uint64_t *__restrict__ chunk_prefix,
                                                     uint64_t capacity, int vec_out,
                                                     uint32_t *__restrict__ trip) {
-  const uint64_t x0 = (uint64_t)blockIdx.x * kEmitCells + threadIdx.x * kEmitPerThread;
-  uint32_t v[16];
-  const uint32_t valid = load16(cells, n_cells, x0, vec != 0, v);
-  uint32_t hits = 0, mask = 0;
+  __shared__ T s_cells[kEmitCells];
+  __shared__ __attribute__((aligned(16))) uint32_t s_stage[4][kEmitStageWords];
+  __shared__ uint32_t s_wave_hits[4];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t xb = (uint64_t)blockIdx.x * kEmitCells;
+  {
+    uint32_t v[16];
+    load16(cells, n_cells, xb + threadIdx.x * kEmitPerThread, vec != 0, v);
 #pragma unroll
-  for (int e = 0; e < 16; ++e)
-    if ((uint32_t)e < valid && v[e] <= thr) {
-      ++hits;
-      mask |= 1u << e;
-    }
-  // The block's triplets are contiguous in the output: they are assembled in
-  // LDS in cell order, then stored with consecutive lanes on consecutive
-  // words.  The block's threads take kEmitRounds turns (the cells of
-  // 256 / kEmitRounds threads each), which keeps the LDS per workgroup at
-  // 3 * 4 * kEmitCells / kEmitRounds bytes.
-  __shared__ uint32_t s_trip[3 * kEmitCells / kEmitRounds];
-  __shared__ uint32_t s_round[kEmitRounds + 1];
-  constexpr uint32_t kRoundThreads = 256 / kEmitRounds;
-  uint64_t total = 0;
-  const uint32_t at = (uint32_t)block_exclusive_scan(hits, &total);
-  if (threadIdx.x % kRoundThreads == 0) s_round[threadIdx.x / kRoundThreads] = at;
-  if (threadIdx.x == 0) s_round[kEmitRounds] = (uint32_t)total;
-  const uint64_t base = local_prefix[blockIdx.x] + chunk_prefix[blockIdx.x / kScanChunk];
-  const uint64_t limit = 3 * capacity;
-  uint64_t i = 0, row_start = 0;
-  if (mask) {
-    i = row_of_cell(x0);
-    row_start = i * (i - 1) / 2;
+    for (int e = 0; e < 16; ++e) s_cells[threadIdx.x * kEmitPerThread + e] = (T)v[e];
   }
-  for (uint32_t r = 0; r < kEmitRounds; ++r) {
-    __syncthreads(); // s_round is written; the previous round's stores read s_trip
-    const uint32_t r0 = s_round[r], r1 = s_round[r + 1];
-    if (threadIdx.x / kRoundThreads == r && mask) {
-      uint32_t a = at - r0;
+  wave_lds_sync(); // each wave reads back only the cells its own lanes wrote
+  const uint64_t xw = xb + (uint64_t)wave * kEmitWaveCells;
+  const T *wc = s_cells + wave * kEmitWaveCells;
+  uint64_t bal[16];
+  uint32_t val[16];
+  uint32_t wave_hits = 0;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const uint64_t x = x0 + e;
-        while (x >= row_start + i) { // next row (rows of length i)
-          row_start += i;
-          ++i;
-        }
-        if ((mask >> e) & 1u) {
-          s_trip[3 * a + 0] = (uint32_t)i;
-          s_trip[3 * a + 1] = (uint32_t)(x - row_start);
-          s_trip[3 * a + 2] = v[e];
-          ++a;
-        }
-      }
-    }
-    __syncthreads();
-    const uint64_t first = 3 * (base + r0);
-    const uint64_t words = first >= limit ? 0 : min<uint64_t>(3ull * (r1 - r0), limit - first);
-    if (!vec_out) {
-      for (uint64_t w = threadIdx.x; w < words; w += 256) trip[first + w] = s_trip[w];
+  for (int e = 0; e < 16; ++e) {
+    val[e] = wc[64 * e + lane];
+    const bool hit = xw + 64 * e + lane < n_cells && val[e] <= thr;
+    bal[e] = __ballot(hit);
+    wave_hits += (uint32_t)__builtin_popcountll(bal[e]);
+  }
+  if (lane == 0) s_wave_hits[wave] = wave_hits;
+  __syncthreads();
+  uint64_t base = local_prefix[blockIdx.x] + chunk_prefix[blockIdx.x / kScanChunk];
+  for (uint32_t w = 0; w < wave; ++w) base += s_wave_hits[w];
+  if (wave_hits == 0) return;
+  // row of this lane's first cell; each step moves 64 cells on
+  uint64_t x = xw + lane;
+  uint64_t i = row_of_cell(x), row_start = i * (i - 1) / 2;
+  const uint64_t limit = 3 * capacity;
+  uint32_t *const st = s_stage[wave];
+#pragma unroll
+  for (int h = 0; h < 16; h += kEmitHalf) {
+    uint32_t turn = 0;
+#pragma unroll
+    for (int e = h; e < h + (int)kEmitHalf; ++e)
+      turn += (uint32_t)__builtin_popcountll(bal[e]);
+    if (turn == 0) {
+      x += 64 * kEmitHalf;
       continue;
     }
-    // 16-byte stores from the first 16-byte boundary of the output on (the
-    // output itself is 16-byte aligned); single words before and after.
-    const uint32_t head = (uint32_t)min<uint64_t>((4u - (uint32_t)(first & 3u)) & 3u, words);
-    const uint64_t quads = (words - head) / 4;
-    if (threadIdx.x < head) trip[first + threadIdx.x] = s_trip[threadIdx.x];
-    uint4 *dst = reinterpret_cast<uint4 *>(trip + first + head);
-    for (uint64_t q = threadIdx.x; q < quads; q += 256) {
-      const uint32_t *src = s_trip + head + 4 * q;
-      dst[q] = make_uint4(src[0], src[1], src[2], src[3]);
+    const uint64_t first = 3 * base;
+    const uint32_t shift = (uint32_t)(first & 3u);
+    uint32_t a = 0;
+#pragma unroll
+    for (int e = h; e < h + (int)kEmitHalf; ++e) {
+      while (x >= row_start + i) { // next row (rows of length i)
+        row_start += i;
+        ++i;
+      }
+      if ((bal[e] >> lane) & 1u) {
+        const uint32_t k =
+            shift + 3 * (a + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[e] >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo(
+                                                           (uint32_t)bal[e], 0u)));
+        st[k] = (uint32_t)i;
+        st[k + 1] = (uint32_t)(x - row_start);
+        st[k + 2] = val[e];
+      }
+      a += (uint32_t)__builtin_popcountll(bal[e]);
+      x += 64;
     }
-    for (uint64_t w = head + 4 * quads + threadIdx.x; w < words; w += 256)
-      trip[first + w] = s_trip[w];
+    wave_lds_sync();
+    const uint64_t words = first >= limit ? 0 : min<uint64_t>(3ull * turn, limit - first);
+    if (!vec_out) {
+      for (uint64_t w = lane; w < words; w += 64) trip[first + w] = st[shift + w];
+    } else {
+      // 16-byte stores from the first 16-byte boundary of the output on (the
+      // output itself is 16-byte aligned); single words before and after.
+      const uint32_t head = (uint32_t)min<uint64_t>((4u - shift) & 3u, words);
+      const uint64_t quads = (words - head) / 4;
+      if (lane < head) trip[first + lane] = st[shift + lane];
+      using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
+      u32x4 *dst = reinterpret_cast<u32x4 *>(trip + first + head);
+      const u32x4 *src = reinterpret_cast<const u32x4 *>(st + shift + head);
+      // streaming stores: the triplets are not read back by this kernel
+      for (uint64_t q = lane; q < quads; q += 64) __builtin_nontemporal_store(src[q], dst + q);
+      for (uint64_t w = head + 4 * quads + lane; w < words; w += 64)
+        trip[first + w] = st[shift + w];
+    }
+    wave_lds_sync(); // the stores have read the stage before the next turn writes it
+    base += turn;
   }
 }
 
-// Emits into device triplets; returns the total number of qualifying cells.
+// Scratch of one emit over n_cells cells: the scan's chunk sums
+// (chunks + 1 uint64) followed by the block counts (blocks uint32).
+static uint64_t emit_blocks(uint64_t n_cells) { return (n_cells + kEmitCells - 1) / kEmitCells; }
+static uint64_t emit_chunks(uint64_t blocks) { return (blocks + kScanChunk - 1) / kScanChunk; }
+static uint64_t emit_scratch_bytes(uint64_t n_cells) {
+  const uint64_t blocks = emit_blocks(n_cells);
+  return blocks ? (emit_chunks(blocks) + 1) * 8 + blocks * 4 : 0;
+}
+
+// Emits into device triplets, with the scratch at `work`
+// (emit_scratch_bytes(n_cells) bytes, 8-byte aligned); returns the total
+// number of qualifying cells.  Every scratch word is written before it is
+// read, so the scratch needs no clearing.
 template <typename T>
-static uint64_t emit_t(const T *d_cells, uint64_t n_cells, uint32_t thr,
-                       uint32_t *d_trip, uint64_t capacity, hipStream_t s) {
-  const uint64_t blocks = (n_cells + kEmitCells - 1) / kEmitCells;
+static uint64_t emit_t(const T *d_cells, uint64_t n_cells, uint32_t thr, uint32_t *d_trip,
+                       uint64_t capacity, void *work, hipStream_t s) {
+  const uint64_t blocks = emit_blocks(n_cells);
   if (!blocks) return 0;
   HD_REQUIRE(blocks < (1ull << 31), "triangle too large for the emit grid");
   const int vec = (reinterpret_cast<uintptr_t>(d_cells) % 16) == 0 ? 1 : 0;
-  const uint64_t chunks = (blocks + kScanChunk - 1) / kScanChunk;
-  DeviceBuffer d_counts(blocks * sizeof(uint32_t)), d_chunks((chunks + 1) * sizeof(uint64_t));
+  const uint64_t chunks = emit_chunks(blocks);
+  uint64_t *d_chunks = static_cast<uint64_t *>(work);
+  uint32_t *d_counts = reinterpret_cast<uint32_t *>(d_chunks + chunks + 1);
   hipLaunchKernelGGL((k_emit_count<T>), dim3((uint32_t)blocks), dim3(256), 0, s, d_cells,
-                     n_cells, thr, vec, d_counts.get<uint32_t>());
+                     n_cells, thr, vec, d_counts);
   HD_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_emit_scan_local, dim3((uint32_t)chunks), dim3(256), 0, s,
-                     d_counts.get<uint32_t>(), blocks, d_chunks.get<uint64_t>());
+  hipLaunchKernelGGL(k_emit_scan_local, dim3((uint32_t)chunks), dim3(256), 0, s, d_counts,
+                     blocks, d_chunks);
   HD_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_emit_scan_chunks, dim3(1), dim3(256), 0, s, d_chunks.get<uint64_t>(),
-                     chunks);
+  hipLaunchKernelGGL(k_emit_scan_chunks, dim3(1), dim3(256), 0, s, d_chunks, chunks);
   HD_HIP(hipGetLastError());
   uint64_t total = 0;
-  HD_HIP(hipMemcpyAsync(&total, d_chunks.get<uint64_t>() + chunks, sizeof total,
-                        hipMemcpyDeviceToHost, s));
+  HD_HIP(hipMemcpyAsync(&total, d_chunks + chunks, sizeof total, hipMemcpyDeviceToHost, s));
   HD_HIP(hipStreamSynchronize(s));
   if (total && capacity) {
     const int vec_out = (reinterpret_cast<uintptr_t>(d_trip) % 16) == 0 ? 1 : 0;
     hipLaunchKernelGGL((k_emit_write<T>), dim3((uint32_t)blocks), dim3(256), 0, s, d_cells,
-                       n_cells, thr, vec, d_counts.get<uint32_t>(), d_chunks.get<uint64_t>(),
-                       capacity, vec_out, d_trip);
+                       n_cells, thr, vec, d_counts, d_chunks, capacity, vec_out, d_trip);
     HD_HIP(hipGetLastError());
     HD_HIP(hipStreamSynchronize(s));
   }
@@ -1260,14 +1299,22 @@ int hd_consensus_diff(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
   });
 }
 
-// Sparse emit of a device triangle into device triplets; returns the total.
+// Sparse emit of a device triangle into device triplets with the scratch at
+// `work` (emit_scratch_bytes(nc) bytes); returns the total.
 static uint64_t emit_dev(const void *d_cells, uint64_t nc, int out_bytes, uint32_t thr,
-                         uint32_t *d_trip, uint64_t capacity, hipStream_t s) {
+                         uint32_t *d_trip, uint64_t capacity, void *work, hipStream_t s) {
   if (out_bytes == 1)
-    return emit_t(static_cast<const uint8_t *>(d_cells), nc, thr, d_trip, capacity, s);
+    return emit_t(static_cast<const uint8_t *>(d_cells), nc, thr, d_trip, capacity, work, s);
   if (out_bytes == 2)
-    return emit_t(static_cast<const uint16_t *>(d_cells), nc, thr, d_trip, capacity, s);
-  return emit_t(static_cast<const uint32_t *>(d_cells), nc, thr, d_trip, capacity, s);
+    return emit_t(static_cast<const uint16_t *>(d_cells), nc, thr, d_trip, capacity, work, s);
+  return emit_t(static_cast<const uint32_t *>(d_cells), nc, thr, d_trip, capacity, work, s);
+}
+
+// The same with the scratch in a device buffer of this call.
+static uint64_t emit_dev_alloc(const void *d_cells, uint64_t nc, int out_bytes, uint32_t thr,
+                               uint32_t *d_trip, uint64_t capacity, hipStream_t s) {
+  DeviceBuffer work(emit_scratch_bytes(nc));
+  return emit_dev(d_cells, nc, out_bytes, thr, d_trip, capacity, work.get(), s);
 }
 
 int hd_sparse_emit(const void *cells, uint64_t n, int out_bytes, int64_t threshold,
@@ -1288,7 +1335,7 @@ int hd_sparse_emit(const void *cells, uint64_t n, int out_bytes, int64_t thresho
     upload(d_cells.get(), cells, nc * out_bytes, s);
     DeviceBuffer d_trip(std::max<uint64_t>(capacity, 1) * 12);
     const uint64_t total =
-        emit_dev(d_cells.get(), nc, out_bytes, thr, d_trip.get<uint32_t>(), capacity, s);
+        emit_dev_alloc(d_cells.get(), nc, out_bytes, thr, d_trip.get<uint32_t>(), capacity, s);
     const uint64_t got = std::min(total, capacity);
     if (got) {
       HD_HIP(hipMemcpyAsync(triplets, d_trip.get(), got * 12, hipMemcpyDeviceToHost, s));
@@ -1314,7 +1361,7 @@ int hd_sparse_emit_dev(const void *d_cells, uint64_t n, int out_bytes, int64_t t
     HD_REQUIRE(d_triplets || capacity == 0, "null triplets");
     ensure_device();
     const uint32_t thr = threshold > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)threshold;
-    const uint64_t total = emit_dev(d_cells, nc, out_bytes, thr, d_triplets, capacity,
+    const uint64_t total = emit_dev_alloc(d_cells, nc, out_bytes, thr, d_triplets, capacity,
                                     (hipStream_t)stream);
     *count = total;
     if (total > capacity)
