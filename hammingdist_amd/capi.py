@@ -55,6 +55,8 @@ SIGNATURES = {
     "hd_sparse_emit": (C.c_int, [vp, C.c_uint64, C.c_int, C.c_int64, vp, C.c_uint64, u64p]),
     "hd_sparse_emit_dev": (C.c_int, [vp, C.c_uint64, C.c_int, C.c_int64, vp, C.c_uint64, u64p,
                                      vp]),
+    "hd_sparse_emit_dev_noalloc": (C.c_int, [vp, C.c_uint64, C.c_int, C.c_int64, vp, C.c_uint64,
+                                             u64p, vp]),
     "hd_planes_layout": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, u64p,
                                    u32p, u32p, u64p]),
     "hd_nibble_mask_dev": (C.c_int, [vp, C.c_uint64, C.c_uint64, u32p, vp]),
@@ -201,6 +203,22 @@ def sparse_emit_dev(d_cells, n: int, out_bytes: int, threshold: int, d_triplets,
     st = lib().hd_sparse_emit_dev(ptr(d_cells), n, out_bytes, threshold,
                                   None if d_triplets is None else ptr(d_triplets), capacity,
                                   C.byref(count), stream)
+    if st not in (HD_OK, HD_ERR_CAPACITY) or (st == HD_ERR_CAPACITY and capacity):
+        check(st)
+    return count.value
+
+
+HD_EMIT_NOALLOC_MAX_CELLS = 1 << 33
+
+
+def sparse_emit_dev_noalloc(d_cells, n: int, out_bytes: int, threshold: int, d_triplets,
+                            capacity: int, stream: int = 0) -> int:
+    """sparse_emit_dev through the entry point that allocates no device
+    memory (its scratch is the library's static device array)."""
+    count = C.c_uint64(0)
+    st = lib().hd_sparse_emit_dev_noalloc(ptr(d_cells), n, out_bytes, threshold,
+                                          None if d_triplets is None else ptr(d_triplets),
+                                          capacity, C.byref(count), stream)
     if st not in (HD_OK, HD_ERR_CAPACITY) or (st == HD_ERR_CAPACITY and capacity):
         check(st)
     return count.value

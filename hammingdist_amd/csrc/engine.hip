@@ -775,11 +775,47 @@ __global__ __launch_bounds__(256) void k_emit_write(const T *__restrict__ cells,
 
 // Scratch of one emit over n_cells cells: the scan's chunk sums
 // (chunks + 1 uint64) followed by the block counts (blocks uint32).
-static uint64_t emit_blocks(uint64_t n_cells) { return (n_cells + kEmitCells - 1) / kEmitCells; }
-static uint64_t emit_chunks(uint64_t blocks) { return (blocks + kScanChunk - 1) / kScanChunk; }
-static uint64_t emit_scratch_bytes(uint64_t n_cells) {
-  const uint64_t blocks = emit_blocks(n_cells);
-  return blocks ? (emit_chunks(blocks) + 1) * 8 + blocks * 4 : 0;
+static constexpr uint64_t emit_blocks(uint64_t n_cells) {
+  return (n_cells + kEmitCells - 1) / kEmitCells;
+}
+static constexpr uint64_t emit_chunks(uint64_t blocks) {
+  return (blocks + kScanChunk - 1) / kScanChunk;
+}
+static constexpr uint64_t emit_scratch_bytes(uint64_t n_cells) {
+  return emit_blocks(n_cells) ? (emit_chunks(emit_blocks(n_cells)) + 1) * 8 +
+                                    emit_blocks(n_cells) * 4
+                              : 0;
+}
+
+// ---------------------------------------------------------------------------
+// the emit's scratch without allocation (hd_sparse_emit_dev_noalloc)
+// ---------------------------------------------------------------------------
+// A file-scope device array sized for HD_EMIT_NOALLOC_MAX_CELLS cells (8 MiB),
+// held by every GPU the library runs on until the process ends.  Host code
+// never uses the array's name as an address: its device address is taken
+// once per device with hipGetSymbolAddress, with that device current, and
+// kept in g_scratch_ptr.  Every call on a GPU shares that GPU's array.
+static constexpr uint64_t kNoAllocScratchWords =
+    (emit_scratch_bytes(HD_EMIT_NOALLOC_MAX_CELLS) + 7) / 8;
+static __device__ __align__(256) uint64_t scratch[kNoAllocScratchWords];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device.
+static void *noalloc_scratch() {
+  int dev = 0;
+  HD_HIP(hipGetDevice(&dev));
+  HD_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    HD_REQUIRE(p, "no device address for the emit scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
 }
 
 // Emits into device triplets, with the scratch at `work`
@@ -1341,6 +1377,33 @@ int hd_sparse_emit(const void *cells, uint64_t n, int out_bytes, int64_t thresho
       HD_HIP(hipMemcpyAsync(triplets, d_trip.get(), got * 12, hipMemcpyDeviceToHost, s));
       HD_HIP(hipStreamSynchronize(s));
     }
+    *count = total;
+    if (total > capacity)
+      throw Error(HD_ERR_CAPACITY, "more cells below the threshold than capacity");
+  });
+}
+
+int hd_sparse_emit_dev_noalloc(const void *d_cells, uint64_t n, int out_bytes,
+                               int64_t threshold, uint32_t *d_triplets, uint64_t capacity,
+                               uint64_t *count, void *stream) {
+  return guarded([&] {
+    HD_REQUIRE(count, "null count");
+    HD_REQUIRE(out_bytes == 1 || out_bytes == 2 || out_bytes == 4,
+               "out_bytes must be 1, 2 or 4");
+    *count = 0;
+    const uint64_t nc = tri_cells(n);
+    if (nc == 0 || threshold < 0) return;
+    HD_REQUIRE(nc <= HD_EMIT_NOALLOC_MAX_CELLS,
+               "triangle of " + std::to_string(nc) +
+                   " cells exceeds the no-allocation emit's maximum of " +
+                   std::to_string((unsigned long long)HD_EMIT_NOALLOC_MAX_CELLS) +
+                   " cells (use hd_sparse_emit_dev)");
+    HD_REQUIRE(d_cells, "null buffer");
+    HD_REQUIRE(d_triplets || capacity == 0, "null triplets");
+    ensure_device();
+    const uint32_t thr = threshold > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)threshold;
+    const uint64_t total = emit_dev(d_cells, nc, out_bytes, thr, d_triplets, capacity,
+                                    noalloc_scratch(), (hipStream_t)stream);
     *count = total;
     if (total > capacity)
       throw Error(HD_ERR_CAPACITY, "more cells below the threshold than capacity");

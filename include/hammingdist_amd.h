@@ -144,6 +144,22 @@ int hd_sparse_emit_dev(const void *d_cells, uint64_t n, int out_bytes,
                        int64_t threshold, uint32_t *d_triplets, uint64_t capacity,
                        uint64_t *count, void *stream);
 
+/* hd_sparse_emit_dev without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, order, count and
+ * capacity rule as hd_sparse_emit_dev.  Its scratch (per 4096-cell block
+ * counts and scan sums) is a static device array of the library sized for
+ * HD_EMIT_NOALLOC_MAX_CELLS cells (8 MiB), held by each GPU the library runs
+ * on until the process ends; every scratch word is written before it is
+ * read, so no clearing is needed.  Triangles of more cells are rejected
+ * (HD_ERR_INVALID) before any device work.  All calls on one GPU share its
+ * array: calls that overlap on one GPU (other threads, other streams) race
+ * and must be serialised by the caller.                                    */
+#define HD_EMIT_NOALLOC_MAX_CELLS 8589934592ull /* 2^33 cells: n <= 131,072 */
+int hd_sparse_emit_dev_noalloc(const void *d_cells, uint64_t n, int out_bytes,
+                               int64_t threshold, uint32_t *d_triplets,
+                               uint64_t capacity, uint64_t *count, void *stream);
+
 /* ---- device-resident API (device pointers, hipStream_t passed as void*) -- */
 /* Layout of the bit-sliced planes buffer an encoded dataset needs:
  * planes[k][p][n_pad] uint32, k < n_words (32 sites each), p < n_planes.    */
