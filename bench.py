@@ -633,9 +633,27 @@ def run_e2e(args, workloads, np):
 
         defer_check("end-to-end text rows (200 cells)", check_e2e)
         cells = n * (n - 1) // 2
+        # the round trip: the same text read back by from_lower_triangular
+        # (parallel parse of the mapped file into the uint8 triangle)
+        t = time.perf_counter()
+        back = hd.from_lower_triangular(out)
+        load_s = time.perf_counter() - t
+        assert back.nsamples == n
+        lt = back.lt_array
+        assert lt.size == cells
+        for i in [1, n - 1]:
+            f0 = i * (i - 1) // 2
+            with open(out, "rb") as f:
+                f.seek(i * (i - 1))
+                row = [int(v) for v in f.read(2 * i).decode()[:-1].split(",")]
+            assert lt[f0:f0 + i].tolist() == row, "from_lower_triangular differs from the text"
+        del back, lt
+        log(f"e2e: from_lower_triangular read {size / 1e9:.2f} GB back in {load_s:.2f}s")
         res = {"seconds": secs, "cells_per_s": cells / secs, "text_GB": size / 1e9,
                "reference_a100_seconds": 20.969, "speedup_vs_reference_a100": 20.969 / secs,
-               "reference_52core_cpu_seconds": 59.748}
+               "reference_52core_cpu_seconds": 59.748,
+               "from_lower_triangular_seconds": load_s,
+               "from_lower_triangular_GB_per_s": size / load_s / 1e9}
         log(f"e2e: from_fasta_to_lower_triangular n={n} in {secs:.2f}s "
             f"({size / 1e9:.2f} GB text)")
         return res

@@ -26,7 +26,7 @@ BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("HD_OFFLOAD_ARCH", "gfx950")
 
 ENGINE_SOURCES = ["engine.hip", "batch.hip", "serve.hip"]
-HOST_SOURCES = ["fasta.cc", "dataset.cc", "writer.cc", "synth.cc", "module.cc"]
+HOST_SOURCES = ["fasta.cc", "dataset.cc", "loaders.cc", "writer.cc", "synth.cc", "module.cc"]
 
 HIPCC_FLAGS = [
     "-O3",

@@ -121,27 +121,6 @@ DataSet<T>::DataSet(std::vector<std::string> &data, bool include_x, bool clear_i
   result = distances<T>(data, include_x, clear_input_data, use_gpu, max_distance);
 }
 
-// include/hamming/hamming.hh:34-55: n = number of lines; row i contributes
-// its first i values; result keeps the reference's n(n+1)/2 size.
-template <typename T> DataSet<T>::DataSet(const std::string &filename) {
-  std::ifstream count_stream(filename);
-  nsamples = std::count(std::istreambuf_iterator<char>(count_stream),
-                        std::istreambuf_iterator<char>(), '\n');
-  result.resize(nsamples * (nsamples + 1) / 2, T{0});
-  std::ifstream stream(filename);
-  std::string line;
-  std::size_t i = 0, current = 0;
-  while (std::getline(stream, line)) {
-    std::istringstream s(line);
-    std::string d;
-    for (std::size_t j = 0; j < current; ++j) {
-      std::getline(s, d, ',');
-      result[i++] = static_cast<T>(std::stoi(d));
-    }
-    ++current;
-  }
-}
-
 // include/hamming/hamming.hh:57-61
 template <typename T> DataSet<T>::DataSet(Cells<T> &&distances) : result{std::move(distances)} {
   const auto x = 8 * result.size() + 1;
@@ -199,28 +178,6 @@ DataSet<DefaultDistIntType> from_stringlist(std::vector<std::string> &data, bool
                                             bool use_gpu, int max_distance) {
   return DataSet<DefaultDistIntType>(data, include_x, false, {}, use_gpu, max_distance);
 }
-
-DataSet<DefaultDistIntType> from_csv(const std::string &filename) {
-  return DataSet<DefaultDistIntType>(filename);
-}
-
-// include/hamming/hamming.hh:152-166
-template <typename T> DataSet<T> from_lower_triangular(const std::string &filename) {
-  Cells<T> distances;
-  std::ifstream stream(filename);
-  std::string line;
-  while (std::getline(stream, line)) {
-    std::istringstream s(line);
-    std::string d;
-    while (s.good()) {
-      std::getline(s, d, ',');
-      distances.push_back(safe_int_cast<T>(std::stoi(d)));
-    }
-  }
-  return DataSet<T>(std::move(distances));
-}
-template DataSet<std::uint8_t> from_lower_triangular<std::uint8_t>(const std::string &);
-template DataSet<std::uint16_t> from_lower_triangular<std::uint16_t>(const std::string &);
 
 namespace {
 
