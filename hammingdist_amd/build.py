@@ -103,6 +103,12 @@ def build_host() -> str:
     py_inc = sysconfig.get_paths()["include"]
     flags = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{INCLUDE}", f"-I{HOST}",
              f"-I{pybind11.get_include()}", f"-I{py_inc}", "-fvisibility=hidden"]
+    # HD_HOST_SANITIZE=address,undefined builds the host layer (g++ only; the
+    # HIP engine is not affected) with those sanitizers for a checking run
+    # (python loaded with LD_PRELOAD of the matching libasan).
+    sanitize = os.environ.get("HD_HOST_SANITIZE", "")
+    san = [f"-fsanitize={sanitize}", "-fno-omit-frame-pointer", "-g"] if sanitize else []
+    flags += san
     objs, jobs = [], []
     for name in HOST_SOURCES:
         src = os.path.join(HOST, name)
@@ -113,7 +119,7 @@ def build_host() -> str:
         list(ex.map(_run, jobs))
     out = os.path.join(HERE, _module_name())
     tmp = out + ".tmp"
-    _run(["g++", "-shared", "-pthread", "-o", tmp, *objs, f"-L{HERE}", "-lhd_engine",
+    _run(["g++", "-shared", "-pthread", *san, "-o", tmp, *objs, f"-L{HERE}", "-lhd_engine",
           "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, out)
     # the module's former name, from older builds of this tree
