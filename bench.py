@@ -427,10 +427,17 @@ def run_large(args, torch, capi, workloads, np):
         step(True, ev)
         torch.cuda.synchronize()
         ee_ms = ev[0].elapsed_time(ev[1])
+        out_gb = cells * out_bytes / 1e9
         res["early_exit"] = {"kernel_ms": ee_ms,
                              "site_comparisons_per_s": site_cmp / (ee_ms / 1e3),
+                             # the product path's output write: the whole uint8
+                             # triangle, once, by the early-exit tile kernel
+                             "output_GB": out_gb,
+                             "output_write_GB_per_s": out_gb / (ee_ms / 1e3),
+                             "output_write_hbm_frac": out_gb * 1e9 / (ee_ms / 1e3) / PEAK_HBM,
                              "note": "same triangle; a tile stops once all its pairs reach "
-                                     "max_distance"}
+                                     "max_distance (votes every 1,024 sites); bound by the "
+                                     "VALU work before the votes pass, not by the write"}
         log(f"large: early-exit kernel {ee_ms:.2f} ms")
 
     if not args.no_sparse:
@@ -483,7 +490,8 @@ def run_wide(args, torch, capi, workloads, np):
 
     def step():
         capi.encode_dev(d_packed, n, bps, sem, mask, d_planes, sh)
-        capi.all_pairs_dev(d_planes, n, bps, sem, mask, 2, 65535, False, -1, d_out, sh)
+        capi.all_pairs_dev(d_planes, n, bps, sem, mask, 2, 65535, False, args.wide_variant,
+                           d_out, sh)
 
     step()
     torch.cuda.synchronize()
@@ -843,6 +851,7 @@ def main() -> int:
     ap.add_argument("--L", type=int, default=29903)
     ap.add_argument("--max-distance", type=int, default=2)
     ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--wide-variant", type=int, default=-1)
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--serve-rows", type=int, default=1_000_000)
