@@ -1,11 +1,18 @@
 """Autotuner of the all-pairs tile shapes.
 
-For a batch it times every compiled batch tile shape (rows per lane, columns
-per wave, waves per workgroup; hd_batch_variant_info) on the batch itself and
-keeps the fastest; for a plain dataset it does the same over the plain tile
-shapes (hd_variant_info) on a sample of rows.  Winners are cached in a JSON
-file keyed by a coarse signature of the work (plane count, cell type, size
-buckets), so later runs of similar work skip the sweep.
+The tuned space is each kernel's compiled tile variants: rows per lane (the
+32-bit words of the row planes each lane holds per step, R), columns per
+wavefront (C) and, for batches, the schedule (persistent workgroups on an
+atomic tile queue, or one workgroup per tile).  The kernels keep no tile in
+LDS (columns are wave-uniform scalar loads), so there is no LDS depth to tune.
+
+For a batch, tune_batch times every batch variant (hd_batch_variant_info) on
+the batch itself and keeps the fastest; for a plain dataset resident on the
+device, tune_plain times every plain variant (hd_variant_info) on a leading
+band of rows of the dataset (a contiguous slice of its triangle).  Winners are
+cached in a JSON file keyed by a signature of the work (plane code, cell type,
+cap, early exit, and log2 buckets of the job count, rows, words per row and
+pairs), so later runs of similar work skip the sweep.
 
 Cache file: $HD_TUNE_CACHE, else ~/.cache/hammingdist_amd/tuning.json.
 """
@@ -48,12 +55,28 @@ def _bucket(x: float) -> int:
     return int(round(math.log2(max(x, 1.0))))
 
 
-def batch_signature(jobs: list[tuple[int, int, int]]) -> str:
+def batch_signature(jobs: list[tuple[int, int, int]], early_exit: bool = True) -> str:
     """jobs: (n, bytes_per_seq, out_bytes) per dataset."""
     pairs = sum(n * (n - 1) // 2 for n, _, _ in jobs)
     work = sum(n * (n - 1) // 2 * bps for n, bps, _ in jobs)
     mean_n = sum(n for n, _, _ in jobs) / max(1, len(jobs))
-    return f"batch:j{_bucket(len(jobs))}:n{_bucket(mean_n)}:p{_bucket(pairs)}:w{_bucket(work)}"
+    mean_w = sum(bps / 16 for _, bps, _ in jobs) / max(1, len(jobs))
+    widths = "".join(sorted({str(ob) for _, _, ob in jobs}))
+    return (f"batch:j{_bucket(len(jobs))}:n{_bucket(mean_n)}:W{_bucket(mean_w)}"
+            f":p{_bucket(pairs)}:w{_bucket(work)}:o{widths}:e{int(bool(early_exit))}")
+
+
+def plain_signature(n: int, bps: int, semantics: int, mask: int, out_bytes: int,
+                    max_dist: int, early_exit: bool) -> str:
+    """Signature of one plain all-pairs launch: the plane code the engine picks
+    (its stored planes), the cell type, the cap (exact below 64, where the
+    early exit stops tiles within the first chunks, else its log2 bucket), and
+    log2 buckets of the rows and words per row."""
+    _, W, P, total = capi.planes_layout(n, bps, semantics, mask)
+    code = f"s{semantics}p{P}{'c' if total > W * P * ((n + 511) // 512 * 512) else ''}"
+    cap = str(max_dist) if max_dist < 64 else f"b{_bucket(max_dist)}"
+    return (f"plain:{code}:n{_bucket(n)}:W{_bucket(W)}:o{out_bytes}:m{cap}"
+            f":e{int(bool(early_exit))}")
 
 
 def _time(fn, sync, reps: int) -> float:
@@ -74,7 +97,7 @@ def tune_batch(batch: "capi.Batch", jobs: list[tuple[int, int, int]], stream: in
         import torch
 
         sync = torch.cuda.synchronize
-    sig = batch_signature(jobs)
+    sig = batch_signature(jobs, early_exit)
     cache = _load() if use_cache else {}
     if use_cache and sig in cache:
         v = int(cache[sig]["variant"])
@@ -86,6 +109,36 @@ def tune_batch(batch: "capi.Batch", jobs: list[tuple[int, int, int]], stream: in
         times[v] = _time(lambda: batch.run(stream), sync, reps)
     best = min(times, key=times.get)
     batch.prepare(best, early_exit)
+    cache[sig] = {"variant": best, "times": {str(k): t for k, t in times.items()}}
+    if use_cache:
+        _store(cache)
+    return best, times
+
+
+def tune_plain(d_planes, n: int, bps: int, semantics: int, mask: int, out_bytes: int,
+               max_dist: int, early_exit: bool, d_out, stream: int = 0, sync=None,
+               reps: int = 2, sample_rows: int = 8192, use_cache: bool = True):
+    """The fastest plain tile variant for this launch; returns (variant,
+    {variant: seconds}).  Each variant is timed on the triangle of the first
+    min(n, sample_rows) rows (the leading band of the same encoded planes,
+    written into the head of d_out), so the sweep costs a fraction of one
+    full launch; d_out must hold at least that band's cells."""
+    if sync is None:
+        import torch
+
+        sync = torch.cuda.synchronize
+    sig = plain_signature(n, bps, semantics, mask, out_bytes, max_dist, early_exit)
+    cache = _load() if use_cache else {}
+    if use_cache and sig in cache:
+        return int(cache[sig]["variant"]), cache[sig].get("times", {})
+    m = min(n, sample_rows)
+    times = {}
+    for v in range(len(capi.variants())):
+        times[v] = _time(lambda: capi.all_pairs_dev_rows(d_planes, n, bps, semantics, mask,
+                                                         out_bytes, max_dist, early_exit, v,
+                                                         0, m, d_out, stream),
+                         sync, reps)
+    best = min(times, key=times.get)
     cache[sig] = {"variant": best, "times": {str(k): t for k, t in times.items()}}
     if use_cache:
         _store(cache)

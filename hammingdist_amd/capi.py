@@ -63,6 +63,9 @@ SIGNATURES = {
     "hd_encode_dev": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, vp, vp]),
     "hd_all_pairs_dev": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, C.c_int,
                                    C.c_uint32, C.c_int, C.c_int, vp, vp]),
+    "hd_all_pairs_dev_rows": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_int, C.c_uint32,
+                                        C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_uint64,
+                                        C.c_uint64, vp, vp]),
     "hd_reference_distances_dev": (C.c_int, [vp, vp, C.c_uint64, C.c_uint64, vp, C.c_uint64,
                                              C.c_int, vp, vp]),
     "hd_batch_create": (C.c_int, [C.POINTER(vp)]),
@@ -254,6 +257,15 @@ def all_pairs_dev(d_planes, n, bps, semantics, mask, out_bytes, max_dist, early_
                   d_out, stream: int = 0):
     check(lib().hd_all_pairs_dev(ptr(d_planes), n, bps, semantics, mask, out_bytes, max_dist,
                                  1 if early_exit else 0, variant, ptr(d_out), stream))
+
+
+def all_pairs_dev_rows(d_planes, n, bps, semantics, mask, out_bytes, max_dist, early_exit,
+                       variant, row_begin, row_end, d_out, stream: int = 0):
+    """Rows [row_begin, row_end) of the triangle into d_out (band-relative
+    cells)."""
+    check(lib().hd_all_pairs_dev_rows(ptr(d_planes), n, bps, semantics, mask, out_bytes,
+                                      max_dist, int(early_exit), variant, row_begin, row_end,
+                                      ptr(d_out), stream))
 
 
 def reference_distances_dev(d_query, d_db, n_db, bps, d_rows, m, semantics, d_out,

@@ -1108,6 +1108,28 @@ int hd_all_pairs_dev(const uint32_t *d_planes, uint64_t n, uint64_t bytes_per_se
   });
 }
 
+int hd_all_pairs_dev_rows(const uint32_t *d_planes, uint64_t n, uint64_t bytes_per_seq,
+                          int semantics, uint32_t nibble_mask, int out_bytes,
+                          uint32_t max_dist, int early_exit, int variant, uint64_t row_begin,
+                          uint64_t row_end, void *d_out, void *stream) {
+  return guarded([&] {
+    ensure_device();
+    check_max_dist(out_bytes, max_dist);
+    const Geometry g = geometry(n, bytes_per_seq, semantics, nibble_mask);
+    HD_REQUIRE(row_begin <= row_end && row_end <= n, "row band outside the dataset");
+    if (n < 2 || row_end <= std::max<uint64_t>(row_begin, 1)) return;
+    const int v = variant < 0 ? default_variant(g, early_exit != 0, max_dist) : variant;
+    HD_REQUIRE(v < kNumVariants, "unknown variant");
+    const uint64_t TI = tile_rows(v);
+    HD_REQUIRE(row_begin % TI == 0 && (row_end % TI == 0 || row_end == n),
+               "row band must start and end on a multiple of the tile's " +
+                   std::to_string(TI) + " rows (or end at n)");
+    const uint64_t first = row_begin ? row_begin * (row_begin - 1) / 2 : 0;
+    run_allpairs(g, d_planes, out_bytes, max_dist, early_exit != 0, v, row_begin / TI,
+                 (row_end + TI - 1) / TI, d_out, first, (hipStream_t)stream);
+  });
+}
+
 int hd_all_pairs(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
                  int semantics, int out_bytes, uint32_t max_dist, void *host_out) {
   return guarded([&] {

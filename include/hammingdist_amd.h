@@ -184,6 +184,19 @@ int hd_all_pairs_dev(const uint32_t *d_planes, uint64_t n,
                      uint32_t nibble_mask, int out_bytes, uint32_t max_dist,
                      int early_exit, int variant, void *d_out, void *stream);
 
+/* Rows [row_begin, row_end) of the same triangle into d_out, cell (i, j) at
+ * index i*(i-1)/2 + j - row_begin*(row_begin-1)/2 (the band's cells are a
+ * contiguous slice of the linear triangle, so bands concatenate in row
+ * order).  row_begin and row_end must be multiples of the tile variant's rows
+ * (64 x rows_per_lane of hd_variant_info; 512 suits every variant), or
+ * row_end = n.  The unit of one GPU's share when the triangle is split across
+ * GPUs by row bands; no counterpart in the reference.                       */
+int hd_all_pairs_dev_rows(const uint32_t *d_planes, uint64_t n,
+                          uint64_t bytes_per_seq, int semantics,
+                          uint32_t nibble_mask, int out_bytes, uint32_t max_dist,
+                          int early_exit, int variant, uint64_t row_begin,
+                          uint64_t row_end, void *d_out, void *stream);
+
 /* Reference distances of a device query against device rows (see above).   */
 int hd_reference_distances_dev(const uint8_t *d_query, const uint8_t *d_db,
                                uint64_t n_db, uint64_t bytes_per_seq,

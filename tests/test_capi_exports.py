@@ -132,3 +132,23 @@ def test_default_tile_shape():
     assert shape(30000, 5000, capi.HD_SEM_DENSE, acgt_gap_inv | (1 << 3), False, 65535) == \
         (2, 16)
     assert shape(30000, 5000, capi.HD_SEM_CHAR_X, 0xFFFF, False, 65535) == (2, 16)
+
+
+def test_tuning_signatures_separate_the_work_they_describe():
+    # host-only: the cache keys tell apart plane codes, caps, cell types,
+    # early exit and sizes (hd_planes_layout needs no device)
+    from hammingdist_amd import autotune
+
+    dense = (1 << 0) | (1 << 1) | (1 << 2) | (1 << 4) | (1 << 8) | (1 << 15)
+    base = autotune.plain_signature(50000, 14952, capi.HD_SEM_DENSE, dense, 1, 2, True)
+    assert base != autotune.plain_signature(50000, 14952, capi.HD_SEM_DENSE, dense, 1, 3, True)
+    assert base != autotune.plain_signature(50000, 14952, capi.HD_SEM_DENSE, dense, 2, 2, True)
+    assert base != autotune.plain_signature(50000, 14952, capi.HD_SEM_DENSE, dense, 1, 2, False)
+    assert base != autotune.plain_signature(50000, 14952, capi.HD_SEM_DENSE,
+                                            dense | (1 << 3), 1, 2, True)  # X: nibble code
+    assert base != autotune.plain_signature(50000, 14952, capi.HD_SEM_SPARSE_X, dense, 1, 2,
+                                            True)
+    assert base == autotune.plain_signature(50001, 14952, capi.HD_SEM_DENSE, dense, 1, 2, True)
+    jobs = [(100, 800, 1), (2000, 15000, 2)]
+    assert autotune.batch_signature(jobs, True) != autotune.batch_signature(jobs, False)
+    assert autotune.batch_signature(jobs) != autotune.batch_signature([(100, 800, 1)] * 2)

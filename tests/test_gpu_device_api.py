@@ -271,3 +271,67 @@ def test_longest_rows_of_the_offset_codes(variant):
             want = O.all_pairs_sparse(chars, False, 4, md)
             got = run_dev(packed, capi.HD_SEM_SPARSE, 4, md, early, variant)
             np.testing.assert_array_equal(got, want, err_msg=f"sparse L={L} md={md}")
+
+
+def encoded(packed, sem):
+    n, bps = packed.shape
+    stream = torch.cuda.current_stream().cuda_stream
+    d_packed = torch.from_numpy(np.ascontiguousarray(packed)).cuda()
+    mask = capi.nibble_mask_dev(d_packed, n, bps, stream)
+    _, _, _, total = capi.planes_layout(n, bps, sem, mask)
+    d_planes = torch.empty(total, dtype=torch.int32, device="cuda")
+    capi.encode_dev(d_packed, n, bps, sem, mask, d_planes, stream)
+    return d_planes, mask
+
+
+@pytest.mark.parametrize("variant", range(5))
+def test_row_bands_concatenate_to_the_triangle(variant, rng):
+    # hd_all_pairs_dev_rows: bands on 512-row boundaries (every variant's tile
+    # rows divide 512) concatenate to the whole triangle; a band that starts
+    # off a tile boundary is rejected before any launch
+    n, L = 1500, 333
+    seqs = random_strings(rng, n, L, "ACGT-N")
+    packed = O.pack(O.as_chars(seqs))
+    bps = packed.shape[1]
+    stream = torch.cuda.current_stream().cuda_stream
+    d_planes, mask = encoded(packed, capi.HD_SEM_DENSE)
+    want = O.all_pairs_dense(packed, 2, 300)
+    parts = []
+    for r0, r1 in [(0, 512), (512, 1024), (1024, n)]:
+        cells = r1 * (r1 - 1) // 2 - (r0 * (r0 - 1) // 2 if r0 else 0)
+        d_band = torch.zeros(cells * 2, dtype=torch.uint8, device="cuda")
+        for early in (False, True):
+            capi.all_pairs_dev_rows(d_planes, n, bps, capi.HD_SEM_DENSE, mask, 2, 300, early,
+                                    variant, r0, r1, d_band, stream)
+            torch.cuda.synchronize()
+            got = d_band.cpu().numpy().view(np.uint16)
+            first = r0 * (r0 - 1) // 2 if r0 else 0
+            np.testing.assert_array_equal(got, want[first:first + cells], err_msg=f"{r0}:{r1}")
+        parts.append(got.copy())
+    np.testing.assert_array_equal(np.concatenate(parts), want)
+    with pytest.raises(capi.EngineError, match="row band"):
+        capi.all_pairs_dev_rows(d_planes, n, bps, capi.HD_SEM_DENSE, mask, 2, 300, False,
+                                variant, 100, 512, d_band, stream)
+
+
+def test_tune_plain_picks_a_variant_that_matches(rng, tmp_path, monkeypatch):
+    from hammingdist_amd import autotune
+
+    monkeypatch.setenv("HD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    n, L = 2600, 700
+    seqs = random_strings(rng, n, L, "ACGT-N")
+    packed = O.pack(O.as_chars(seqs))
+    bps = packed.shape[1]
+    stream = torch.cuda.current_stream().cuda_stream
+    d_planes, mask = encoded(packed, capi.HD_SEM_DENSE)
+    d_out = torch.zeros(n * (n - 1) // 2, dtype=torch.uint8, device="cuda")
+    v, times = autotune.tune_plain(d_planes, n, bps, capi.HD_SEM_DENSE, mask, 1, 255, True,
+                                   d_out, stream, sample_rows=1024)
+    assert 0 <= v < len(capi.variants()) and len(times) == len(capi.variants())
+    # cached: the second call returns the same variant without a sweep
+    assert autotune.tune_plain(d_planes, n, bps, capi.HD_SEM_DENSE, mask, 1, 255, True,
+                               d_out, stream, sample_rows=1024)[0] == v
+    capi.all_pairs_dev(d_planes, n, bps, capi.HD_SEM_DENSE, mask, 1, 255, True, v, d_out,
+                       stream)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d_out.cpu().numpy(), O.all_pairs_dense(packed, 1, 255))
