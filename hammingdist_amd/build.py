@@ -28,6 +28,10 @@ ARCH = os.environ.get("HD_OFFLOAD_ARCH", "gfx950")
 ENGINE_SOURCES = ["engine.hip", "batch.hip", "serve.hip"]
 HOST_SOURCES = ["fasta.cc", "dataset.cc", "loaders.cc", "writer.cc", "synth.cc", "module.cc"]
 
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+# roctx ranges of the host-side phases (rocprofv3 --marker-trace)
+ROCTX_LINK = [f"-L{ROCM}/lib", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM}/lib"]
+
 HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
@@ -76,7 +80,7 @@ def build_engine() -> str:
     out = os.path.join(HERE, "libhd_engine.so")
     tmp = out + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
-          "-Wl,-soname,libhd_engine.so"])
+          "-Wl,-soname,libhd_engine.so", *ROCTX_LINK])
     os.replace(tmp, out)
     return out
 
@@ -102,6 +106,7 @@ def build_host() -> str:
     os.makedirs(BUILD, exist_ok=True)
     py_inc = sysconfig.get_paths()["include"]
     flags = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{INCLUDE}", f"-I{HOST}",
+             f"-I{ROCM}/include",
              f"-I{pybind11.get_include()}", f"-I{py_inc}", "-fvisibility=hidden"]
     # HD_HOST_SANITIZE=address,undefined builds the host layer (g++ only; the
     # HIP engine is not affected) with those sanitizers for a checking run
@@ -120,7 +125,7 @@ def build_host() -> str:
     out = os.path.join(HERE, _module_name())
     tmp = out + ".tmp"
     _run(["g++", "-shared", "-pthread", *san, "-o", tmp, *objs, f"-L{HERE}", "-lhd_engine",
-          "-Wl,-rpath,$ORIGIN"])
+          "-Wl,-rpath,$ORIGIN", *ROCTX_LINK])
     os.replace(tmp, out)
     # the module's former name, from older builds of this tree
     for name in os.listdir(HERE):

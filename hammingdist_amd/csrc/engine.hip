@@ -924,6 +924,7 @@ struct ReadyMark {
 // freshly allocated destination in parallel).
 static void download(void *host, const void *dev, uint64_t bytes, hipStream_t s,
                      const std::vector<ReadyMark> *marks = nullptr) {
+  const TraceRange trace("download");
   if (bytes <= kStage) {
     if (marks && !marks->empty()) HD_HIP(hipStreamWaitEvent(s, marks->back().ev, 0));
     HD_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s));
@@ -959,6 +960,7 @@ static void download(void *host, const void *dev, uint64_t bytes, hipStream_t s,
 // fill chunk c + 1 while the DMA of chunk c runs.  Returns once every chunk
 // has reached the device.
 static void upload(void *dev, const void *host, uint64_t bytes, hipStream_t s) {
+  const TraceRange trace("upload");
   if (bytes <= kStage) {
     HD_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s));
     return;
@@ -988,6 +990,7 @@ struct Encoded {
 
 static void upload_encode(Encoded &e, const uint8_t *packed, uint64_t n,
                           uint64_t bpr, int semantics, hipStream_t s) {
+  const TraceRange trace("upload + encode");
   geometry(n, bpr, HD_SEM_DENSE, 0); // argument checks
   e.packed.alloc(n * bpr);
   upload(e.packed.get(), packed, n * bpr, s);
@@ -1084,6 +1087,7 @@ int hd_encode_dev(const uint8_t *d_packed, uint64_t n, uint64_t bytes_per_seq,
                   int semantics, uint32_t nibble_mask, uint32_t *d_planes,
                   void *stream) {
   return guarded([&] {
+    const TraceRange trace("hd_encode_dev");
     ensure_device();
     const Geometry g = geometry(n, bytes_per_seq, semantics, nibble_mask);
     launch_encode(g, d_packed, d_planes, (hipStream_t)stream);
@@ -1095,6 +1099,7 @@ int hd_all_pairs_dev(const uint32_t *d_planes, uint64_t n, uint64_t bytes_per_se
                      uint32_t max_dist, int early_exit, int variant, void *d_out,
                      void *stream) {
   return guarded([&] {
+    const TraceRange trace("hd_all_pairs_dev");
     ensure_device();
     check_max_dist(out_bytes, max_dist);
     const Geometry g = geometry(n, bytes_per_seq, semantics, nibble_mask);
@@ -1113,6 +1118,7 @@ int hd_all_pairs_dev_rows(const uint32_t *d_planes, uint64_t n, uint64_t bytes_p
                           uint32_t max_dist, int early_exit, int variant, uint64_t row_begin,
                           uint64_t row_end, void *d_out, void *stream) {
   return guarded([&] {
+    const TraceRange trace("hd_all_pairs_dev_rows");
     ensure_device();
     check_max_dist(out_bytes, max_dist);
     const Geometry g = geometry(n, bytes_per_seq, semantics, nibble_mask);
@@ -1133,6 +1139,7 @@ int hd_all_pairs_dev_rows(const uint32_t *d_planes, uint64_t n, uint64_t bytes_p
 int hd_all_pairs(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
                  int semantics, int out_bytes, uint32_t max_dist, void *host_out) {
   return guarded([&] {
+    const TraceRange trace("hd_all_pairs");
     check_max_dist(out_bytes, max_dist);
     if (n < 2) return;
     HD_REQUIRE(packed && host_out, "null buffer");
@@ -1179,6 +1186,7 @@ int hd_all_pairs_stream(const uint8_t *packed, uint64_t n, uint64_t bytes_per_se
                         int semantics, int out_bytes, uint32_t max_dist,
                         uint64_t band_cells, hd_band_cb cb, void *user) {
   return guarded([&] {
+    const TraceRange trace("hd_all_pairs_stream");
     check_max_dist(out_bytes, max_dist);
     HD_REQUIRE(cb, "callback required");
     if (n < 2) return;
@@ -1235,10 +1243,14 @@ int hd_all_pairs_stream(const uint8_t *packed, uint64_t n, uint64_t bytes_per_se
     for (size_t b = 0; b < std::min<size_t>(2, n_bands); ++b) enqueue(b);
     for (size_t b = 0; b < n_bands; ++b) {
       const int slot = (int)(b & 1);
-      HD_HIP(hipEventSynchronize(ev_c[slot].get()));
+      {
+        const TraceRange wait("wait for band");
+        HD_HIP(hipEventSynchronize(ev_c[slot].get()));
+      }
       uint64_t r0, r1;
       band_range(b, r0, r1);
       const uint64_t c0 = tri_cells(r0), nc = tri_cells(r1) - c0;
+      const TraceRange band("band callback");
       if (nc && cb(user, hbuf[slot].get(), c0, nc, r0 == 0 ? 1 : r0, r1) != 0) {
         HD_HIP(hipStreamSynchronize(s));
         HD_HIP(hipStreamSynchronize(copy.get()));
@@ -1254,6 +1266,7 @@ int hd_all_pairs_stream(const uint8_t *packed, uint64_t n, uint64_t bytes_per_se
 int hd_distance_pair(const uint8_t *a, const uint8_t *b, uint64_t bytes,
                      int semantics, int32_t max_dist, int32_t *out) {
   return guarded([&] {
+    const TraceRange trace("hd_distance_pair");
     HD_REQUIRE(a && b && out, "null buffer");
     HD_REQUIRE(max_dist >= 0, "max_dist must be >= 0");
     std::vector<uint8_t> both(2 * bytes);
@@ -1285,6 +1298,7 @@ int hd_reference_distances(const uint8_t *query, const uint8_t *db, uint64_t n_d
                            uint64_t bytes_per_seq, const uint64_t *rows, uint64_t m,
                            int semantics, uint32_t *out) {
   return guarded([&] {
+    const TraceRange trace("hd_reference_distances");
     HD_REQUIRE(bytes_per_seq > 0 && bytes_per_seq % 8 == 0,
                "bytes_per_seq must be a positive multiple of 8");
     HD_REQUIRE(semantics == HD_SEM_DENSE || semantics == HD_SEM_CHAR_X,
@@ -1317,6 +1331,7 @@ int hd_reference_distances(const uint8_t *query, const uint8_t *db, uint64_t n_d
 int hd_consensus_diff(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
                       uint64_t L, uint64_t *n_diff, uint32_t *nibble_mask) {
   return guarded([&] {
+    const TraceRange trace("hd_consensus_diff");
     HD_REQUIRE(bytes_per_seq > 0 && bytes_per_seq % 8 == 0,
                "bytes_per_seq must be a positive multiple of 8");
     HD_REQUIRE(L <= 2 * bytes_per_seq, "L exceeds the packed row length");
@@ -1378,6 +1393,7 @@ static uint64_t emit_dev_alloc(const void *d_cells, uint64_t nc, int out_bytes, 
 int hd_sparse_emit(const void *cells, uint64_t n, int out_bytes, int64_t threshold,
                    uint32_t *triplets, uint64_t capacity, uint64_t *count) {
   return guarded([&] {
+    const TraceRange trace("hd_sparse_emit");
     HD_REQUIRE(count, "null count");
     HD_REQUIRE(out_bytes == 1 || out_bytes == 2 || out_bytes == 4,
                "out_bytes must be 1, 2 or 4");
@@ -1409,6 +1425,7 @@ int hd_sparse_emit_dev_noalloc(const void *d_cells, uint64_t n, int out_bytes,
                                int64_t threshold, uint32_t *d_triplets, uint64_t capacity,
                                uint64_t *count, void *stream) {
   return guarded([&] {
+    const TraceRange trace("hd_sparse_emit_dev_noalloc");
     HD_REQUIRE(count, "null count");
     HD_REQUIRE(out_bytes == 1 || out_bytes == 2 || out_bytes == 4,
                "out_bytes must be 1, 2 or 4");
@@ -1436,6 +1453,7 @@ int hd_sparse_emit_dev(const void *d_cells, uint64_t n, int out_bytes, int64_t t
                        uint32_t *d_triplets, uint64_t capacity, uint64_t *count,
                        void *stream) {
   return guarded([&] {
+    const TraceRange trace("hd_sparse_emit_dev");
     HD_REQUIRE(count, "null count");
     HD_REQUIRE(out_bytes == 1 || out_bytes == 2 || out_bytes == 4,
                "out_bytes must be 1, 2 or 4");

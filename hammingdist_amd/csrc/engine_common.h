@@ -9,10 +9,24 @@
 #include <stdexcept>
 #include <string>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "hammingdist_amd.h"
 #include "kernels.h"
 
 namespace hd {
+
+// A roctx range over a scope: the engine's host-side phases (uploads,
+// encodes, launches, downloads, callbacks) show up by name in
+// `rocprofv3 --marker-trace` beside the kernels; without a profiler a range
+// costs two calls into the roctx stub.
+class TraceRange {
+public:
+  explicit TraceRange(const char *name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange &) = delete;
+  TraceRange &operator=(const TraceRange &) = delete;
+};
 
 // Error carried from an internal failure to the C ABI status code.
 struct Error : std::runtime_error {

@@ -178,6 +178,7 @@ inline std::uint64_t mix64(std::uint64_t h, std::uint64_t w) {
 // ---- FastaRecords ----------------------------------------------------------
 
 FastaRecords::FastaRecords(const std::string &filename, std::size_t n) {
+  const TraceRange trace("read FASTA");
   fd_ = ::open(filename.c_str(), O_RDONLY);
   if (fd_ < 0) throw std::runtime_error("Error: Failed to open file '" + filename + "'");
   struct stat st {};
@@ -300,6 +301,7 @@ bool FastaRecords::same_text(std::size_t a, std::size_t b) const {
 }
 
 std::vector<std::size_t> FastaRecords::dedup(std::vector<std::size_t> &unique) const {
+  const TraceRange trace("dedup");
   std::vector<std::uint64_t> hashes(spans_.size());
   parallel_for(spans_.size(), [&](std::uint64_t b, std::uint64_t e) {
     for (std::uint64_t k = b; k < e; ++k) hashes[k] = hash(k);
@@ -328,6 +330,7 @@ std::vector<std::size_t> FastaRecords::dedup(std::vector<std::size_t> &unique) c
 
 void FastaRecords::pack_rows(const std::vector<std::size_t> &records, const PackTable &t,
                              GeneBlock *out, std::uint64_t bps) const {
+  const TraceRange trace("pack rows");
   parallel_for(records.size(), [&](std::uint64_t b, std::uint64_t e) {
     std::string scratch;
     for (std::uint64_t r = b; r < e; ++r) pack(records[r], t, out + r * bps, bps, scratch);

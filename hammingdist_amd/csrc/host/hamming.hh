@@ -13,9 +13,23 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "hammingdist_amd.h"
 
 namespace hdhost {
+
+// A roctx range over a scope: the host layer's phases (FASTA ingest, packing,
+// text formatting and writing, loading) show up by name in
+// `rocprofv3 --marker-trace`; without a profiler a range costs two calls into
+// the roctx stub.
+class TraceRange {
+public:
+  explicit TraceRange(const char *name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange &) = delete;
+  TraceRange &operator=(const TraceRange &) = delete;
+};
 
 using GeneBlock = std::uint8_t;
 using ReferenceDistIntType = std::uint32_t;

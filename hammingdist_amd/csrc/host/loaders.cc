@@ -228,6 +228,7 @@ void throw_first(const std::vector<RangeError> &errs) {
 } // namespace
 
 template <typename T> DataSet<T>::DataSet(const std::string &filename) {
+  const TraceRange trace("from_csv parse");
   const FileText text(filename);
   const Lines L = find_lines(text.data(), text.size());
   nsamples = L.newlines;
@@ -271,6 +272,7 @@ DataSet<DefaultDistIntType> from_csv(const std::string &filename) {
 }
 
 template <typename T> DataSet<T> from_lower_triangular(const std::string &filename) {
+  const TraceRange trace("from_lower_triangular parse");
   const FileText text(filename);
   const Lines L = find_lines(text.data(), text.size());
   const std::uint64_t lines = L.count();

@@ -172,6 +172,7 @@ void RowWriter::write_raw(const char *data, std::size_t len) {
 template <typename T>
 void RowWriter::write(const T *cells, std::uint64_t row_begin, std::uint64_t row_end,
                       std::uint32_t max_value) {
+  const TraceRange trace("format + write rows");
   if (row_begin == 0) row_begin = 1; // row 0 has no cells
   if (row_end <= row_begin) return;
   const std::uint64_t c0 = row_begin * (row_begin - 1) / 2;
