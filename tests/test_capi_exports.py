@@ -152,3 +152,19 @@ def test_tuning_signatures_separate_the_work_they_describe():
     jobs = [(100, 800, 1), (2000, 15000, 2)]
     assert autotune.batch_signature(jobs, True) != autotune.batch_signature(jobs, False)
     assert autotune.batch_signature(jobs) != autotune.batch_signature([(100, 800, 1)] * 2)
+
+
+def test_row_bands_partition_the_triangle():
+    # the sharding rule: contiguous, aligned, covering, about equal cells
+    from hammingdist_amd import shard
+
+    for n in (2, 511, 512, 5000, 50_000, 100_000):
+        for g in (1, 2, 3, 8):
+            bands = shard.row_bands(n, g)
+            assert bands[0][0] == 0 and bands[-1][1] == n
+            assert all(a[1] == b[0] for a, b in zip(bands, bands[1:]))
+            assert all(r0 % shard.ALIGN == 0 or r0 == n for r0, _ in bands)
+            cells = [shard.band_cells(*b) for b in bands]
+            assert sum(cells) == n * (n - 1) // 2
+            if n >= 50_000:
+                assert max(cells) / (sum(cells) / g) < 1.02

@@ -335,3 +335,29 @@ def test_tune_plain_picks_a_variant_that_matches(rng, tmp_path, monkeypatch):
                        stream)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(d_out.cpu().numpy(), O.all_pairs_dense(packed, 1, 255))
+
+
+@pytest.mark.parametrize("gpus", [2, 3, 8])
+def test_shard_bands_on_one_gpu_equal_the_triangle(gpus, rng):
+    # each band as one GPU would compute it (hd_all_pairs_dev_rows over the
+    # same planes), run here one after another: their concatenation is the
+    # triangle
+    from hammingdist_amd import shard
+
+    n, L = 2100, 200
+    seqs = random_strings(rng, n, L, "ACGT-N")
+    packed = O.pack(O.as_chars(seqs))
+    bps = packed.shape[1]
+    stream = torch.cuda.current_stream().cuda_stream
+    d_planes, mask = encoded(packed, capi.HD_SEM_DENSE)
+    parts = []
+    for r0, r1 in shard.row_bands(n, gpus):
+        cells = shard.band_cells(r0, r1)
+        if cells == 0:
+            continue
+        d_band = torch.empty(cells, dtype=torch.uint8, device="cuda")
+        capi.all_pairs_dev_rows(d_planes, n, bps, capi.HD_SEM_DENSE, mask, 1, 255, True, -1,
+                                r0, r1, d_band, stream)
+        torch.cuda.synchronize()
+        parts.append(d_band.cpu().numpy())
+    np.testing.assert_array_equal(np.concatenate(parts), O.all_pairs_dense(packed, 1, 255))
