@@ -24,10 +24,17 @@ def row_bands(n: int, gpus: int, align: int = ALIGN) -> list[tuple[int, int]]:
     n is small."""
     if gpus < 1:
         raise ValueError("gpus must be >= 1")
+    total = band_cells(0, n)
     cuts = [0]
     for k in range(1, gpus):
-        r = int(round(n * math.sqrt(k / gpus) / align)) * align
-        cuts.append(min(max(r, cuts[-1]), n))
+        # the aligned row nearest (in cells) to the k/G point of the triangle
+        r = n * math.sqrt(k / gpus)
+        lo = int(r // align) * align
+        hi = lo + align
+        target = total * k / gpus
+        best = min((c for c in (lo, hi) if c <= n),
+                   key=lambda c: abs(band_cells(0, c) - target), default=lo)
+        cuts.append(min(max(best, cuts[-1]), n))
     cuts.append(n)
     return [(cuts[k], cuts[k + 1]) for k in range(gpus)]
 

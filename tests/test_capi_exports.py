@@ -167,4 +167,5 @@ def test_row_bands_partition_the_triangle():
             cells = [shard.band_cells(*b) for b in bands]
             assert sum(cells) == n * (n - 1) // 2
             if n >= 50_000:
-                assert max(cells) / (sum(cells) / g) < 1.05
+                # 512-row boundaries: a block near the end is a few % of a band
+                assert max(cells) / (sum(cells) / g) < (1.05 if g <= 3 else 1.10)
