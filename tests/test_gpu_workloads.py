@@ -7,6 +7,8 @@ large  n=50,000, L=29,903, 64-clade generator: from_fasta_to_lower_triangular
 wide   n=100,000, L=10,000, 5% substitutions + 0.1% X: from_fasta_large with
        include_x=True (the sparse route, src/hamming_impl_t.cc:106-180) and
        fasta_reference_distances against one fresh mutant.
+batched 256 datasets of uneven n and L in one tuned launch: every job's
+       triangle against the oracle.
 
 Each workload runs once at a size the oracle finishes in seconds (every cell,
 every byte of the text compared) and once at full size, where cells sampled
@@ -293,3 +295,45 @@ def test_short_rows_past_2_32_sparse_sampled():
     np.testing.assert_array_equal(got, np.array(want))
     d16e = device_triangle(packed, capi.HD_SEM_SPARSE_X, 2, 65535, True)
     assert torch.equal(d16e, d16)
+
+
+# ---- batched: 256 uneven datasets, every job against the oracle --------------
+
+def test_batched_workload_every_job(tmp_path, monkeypatch):
+    # 256 datasets of the plain generator, n log-uniform in [50, 5,000], L
+    # uniform in [1,000, 30,000], alternating uint8 (cap 255) and uint16 (cap
+    # 65,535), each on the reference CPU route's semantics (the oracle's
+    # sparse/dense heuristic); one tuned, graph-captured launch and the
+    # sequential plain launches both give the oracle's triangle for every job
+    from hammingdist_amd import autotune
+
+    monkeypatch.setenv("HD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    specs = workloads.batched()
+    b = capi.Batch()
+    jobs, wants = [], []
+    for spec, ob in specs:
+        chars = workloads.generate_chars(spec)
+        packed = O.pack(chars)
+        n, bps = packed.shape
+        md = 255 if ob == 1 else 65535
+        sparse = O.use_sparse(chars, False)
+        # the engine's consensus count gives the same route
+        n_diff, _ = capi.consensus_diff(packed, n, bps, spec.L)
+        assert (n_diff / (n * spec.L) < 0.005) == sparse
+        if sparse:
+            want = O.all_pairs_sparse(chars, False, ob, md)
+        else:
+            want = O.all_pairs_dense_simd(packed, 0, n, ob, md)
+        b.add(packed, capi.HD_SEM_SPARSE if sparse else capi.HD_SEM_DENSE, ob, md)
+        jobs.append((n, bps, ob))
+        wants.append(want)
+    log(f"batched: {len(jobs)} datasets, {sum(w.size for w in wants)} cells")
+    sh = stream()
+    variant, _ = autotune.tune_batch(b, jobs, sh, torch.cuda.synchronize, reps=1)
+    for run in (b.run, b.run_sequential):
+        run(sh)
+        torch.cuda.synchronize()
+        for k, want in enumerate(wants):
+            np.testing.assert_array_equal(b.output(k), want,
+                                          err_msg=f"job {k} variant {variant} {run.__name__}")
+    b.close()
