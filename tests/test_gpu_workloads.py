@@ -9,6 +9,8 @@ wide   n=100,000, L=10,000, 5% substitutions + 0.1% X: from_fasta_large with
        fasta_reference_distances against one fresh mutant.
 batched 256 datasets of uneven n and L in one tuned launch: every job's
        triangle against the oracle.
+serving the 1,000,000-row database resident in HBM under the bench's request
+       mix: alone == mixed, sampled results against the oracle.
 
 Each workload runs once at a size the oracle finishes in seconds (every cell,
 every byte of the text compared) and once at full size, where cells sampled
@@ -337,3 +339,43 @@ def test_batched_workload_every_job(tmp_path, monkeypatch):
             np.testing.assert_array_equal(b.output(k), want,
                                           err_msg=f"job {k} variant {variant} {run.__name__}")
     b.close()
+
+
+# ---- serving: the 1,000,000-row database resident in HBM ---------------------
+
+def test_serving_workload_full_database():
+    # the serving database at its real size (1,000,000 x 29,903 sites, 15 GB)
+    # and the bench's request mix (90 % queries against 10^4-10^6 random rows,
+    # 10 % all-pairs jobs of 100-3,000 rows): every result the same alone and
+    # in a concurrent mix, sampled rows and cells against the oracle, and the
+    # last database rows reached
+    from hammingdist_amd import serving
+
+    spec = workloads.serving_db()
+    db = workloads.generate_packed(spec)
+    log(f"serving full: db {db.shape}")
+    srv = capi.Server(db, device_bytes=24 << 30, host_bytes=4 << 30, streams=3, max_batch=32)
+    reqs = serving.make_requests(db, spec.L, 60, seed=4004)
+    tail = db.shape[0] - np.arange(1, 65, dtype=np.uint64)
+    reqs.append(serving.Request("query", tail, db[5].copy()))
+    alone = serving.run_alone(srv, reqs)
+    mixed, _, _ = serving.closed_loop(srv, reqs)
+    for a, b in zip(alone, mixed):
+        np.testing.assert_array_equal(a, b)
+    rng = np.random.default_rng(4005)
+    for r, got in zip(reqs, alone):
+        if r.kind == "query":
+            assert got.shape == (r.rows.size,)
+            idx = np.unique(np.concatenate([rng.integers(0, r.rows.size, 64),
+                                            [0, r.rows.size - 1]]))
+            for k in idx:
+                assert got[k] == O.distance_dense(r.query, db[int(r.rows[k])]), (r.rows.size, k)
+        else:
+            n = r.rows.size
+            assert got.shape == (n * (n - 1) // 2,)
+            x = O.sample_cells(rng, n, 64)
+            I, J = O.cell_rows(x)
+            for xi, i, j in zip(x, I, J):
+                want = min(O.distance_dense(db[int(r.rows[i])], db[int(r.rows[j])]), r.max_dist)
+                assert got[int(xi)] == want, (n, i, j)
+    srv.close()
