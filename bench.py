@@ -861,6 +861,9 @@ def run_serving(args, torch, capi, workloads, np):
            "closed_loop": serving.percentiles(lat_c),
            "offered_rate": rate, "achieved_requests_per_s": len(reqs) / secs_o,
            "open_loop": serving.percentiles(lat_o),
+           "open_loop_by_kind": {
+               kind: serving.percentiles([x for x, r in zip(lat_o, reqs) if r.kind == kind])
+               for kind in ("query", "all_pairs")},
            "query_rows_GB_per_s_closed": rows_scanned * db.shape[1] / secs / 1e9,
            "closed_loop_seconds": secs, "closed_loop_submit_seconds": submit_s,
            "open_loop_seconds": secs_o,

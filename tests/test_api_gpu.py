@@ -214,6 +214,38 @@ def test_cpu_gpu_route_consistency(tmp_path, rng):
                 np.testing.assert_array_equal(back.lt_array, ds16.lt_array)
 
 
+def test_from_fasta_cpu_gpu_grid(tmp_path):
+    # src/hamming_t.cc:375-410: from_fasta<uint8_t> and <uint16_t> on uniform
+    # ACGT- sequences (src/tests.cc:7-20), with and without duplicate
+    # removal, over its whole grid of lengths, caps and sample counts; both
+    # routes, every cell, against the oracle.
+    rng = np.random.default_rng(12345)
+    p = str(tmp_path / "grid.fa")
+    calls = 0
+    for remove_duplicates in (False, True):
+        for n_chars in (17, 88, 381, 1023):
+            for md in (0, 1, 2, 3, 89, 497, 9999999):
+                for n_samples in (2, 3, 4, 7, 11, 127, 128, 255, 256, 257, 703):
+                    seqs = random_strings(rng, n_samples, n_chars, "ACGT-")
+                    if remove_duplicates and n_samples > 3:
+                        # a few repeats, so that removal has work to do
+                        seqs[n_samples - 1] = seqs[0]
+                        seqs[n_samples // 2] = seqs[1]
+                    write_fasta(p, seqs)
+                    data, _ = read_fasta(p, remove_duplicates, 0)
+                    for fn, ob in ((hd.from_fasta, 1), (hd.from_fasta_large, 2)):
+                        for use_gpu in (False, True):
+                            ds = fn(p, False, remove_duplicates, 0, use_gpu, md)
+                            want = expected_cells(data, False, use_gpu, ob, md)
+                            assert ds.lt_array.dtype == (np.uint8 if ob == 1 else np.uint16)
+                            np.testing.assert_array_equal(
+                                ds.lt_array, want,
+                                err_msg=f"rd={remove_duplicates} L={n_chars} md={md} "
+                                        f"n={n_samples} bytes={ob} gpu={use_gpu}")
+                            calls += 1
+    assert calls == 2 * 4 * 7 * 11 * 4
+
+
 def test_low_diversity_takes_sparse_semantics():
     # 0.2% of sites differ from the consensus, with 'N's: the CPU route
     # (use_gpu=False) is the sparse semantics, the GPU route is dense.
