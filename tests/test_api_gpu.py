@@ -243,7 +243,18 @@ def test_from_fasta_cpu_gpu_grid(tmp_path):
                                 err_msg=f"rd={remove_duplicates} L={n_chars} md={md} "
                                         f"n={n_samples} bytes={ob} gpu={use_gpu}")
                             calls += 1
-    assert calls == 2 * 4 * 7 * 11 * 4
+                    if n_chars != 1023 and md != 497:
+                        # src/hamming_t.cc:411-451: the streamed lower
+                        # triangle, read back as uint16, equals from_fasta<uint16_t>
+                        out = str(tmp_path / "grid_lt.txt")
+                        hd.from_fasta_to_lower_triangular(p, out, remove_duplicates, 0, True, md)
+                        back = hd.from_lower_triangular_large(out)
+                        np.testing.assert_array_equal(
+                            back.lt_array, expected_cells(data, False, False, 2, md),
+                            err_msg=f"streamed: rd={remove_duplicates} L={n_chars} md={md} "
+                                    f"n={n_samples}")
+                        calls += 1
+    assert calls == 2 * 4 * 7 * 11 * 4 + 2 * 3 * 6 * 11
 
 
 def test_low_diversity_takes_sparse_semantics():
