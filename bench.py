@@ -805,15 +805,27 @@ def run_serving(args, torch, capi, workloads, np):
     scan_ms = Timer(torch).time(scan, 10)
     scan_got = d_rd[-64:].cpu().numpy().astype(np.uint32)
     scan_rows, scan_q = db[-64:].copy(), db[7].copy()
-    del d_db, d_rd, q
-    torch.cuda.empty_cache()
     scan_bytes = n_db * bps + bps + 4 * n_db
     log(f"serving: one query over all {n_db} rows: {scan_ms:.3f} ms, "
         f"{scan_bytes / (scan_ms / 1e3) / 1e9:.0f} GB/s")
-    t = time.perf_counter()
-    srv = capi.Server(db, device_bytes=24 << 30, host_bytes=4 << 30, streams=3, max_batch=32)
-    log(f"serving: db resident in HBM after {time.perf_counter() - t:.1f}s")
     reqs = serving.make_requests(db, spec.L, args.serve_requests, seed=1004)
+    # the sequential baseline: the first requests one at a time through the
+    # plain device entry points, each result downloaded before the next starts
+    n_seq = min(len(reqs), args.serve_sequential)
+    seq_out = serving.run_sequential_plain(capi, torch, d_db, n_db, bps, reqs[:n_seq], sh)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    seq_out = serving.run_sequential_plain(capi, torch, d_db, n_db, bps, reqs[:n_seq], sh)
+    seq_secs = time.perf_counter() - t
+    del d_db, d_rd, q
+    torch.cuda.empty_cache()
+    log(f"serving: {n_seq} requests sequentially through the plain entry points in "
+        f"{seq_secs * 1e3:.0f} ms ({n_seq / seq_secs:.0f} req/s)")
+    t = time.perf_counter()
+    # 16 GiB of pinned host memory holds the inputs and results of the whole
+    # burst of the closed loop (about 12 GB)
+    srv = capi.Server(db, device_bytes=24 << 30, host_bytes=16 << 30, streams=3, max_batch=32)
+    log(f"serving: db resident in HBM after {time.perf_counter() - t:.1f}s")
     # a few requests alone, sampled for the oracle
     sample = []
     for r in reqs[:8]:
@@ -841,13 +853,21 @@ def run_serving(args, torch, capi, workloads, np):
     defer_check("serving: 8 requests alone", check_serving)
     # capacity: everything submitted at once
     warm = reqs[: max(8, len(reqs) // 10)]
-    serving.closed_loop(srv, warm)
-    _, secs, lat_c = serving.closed_loop(srv, reqs)
+    serving.closed_loop(srv, warm, submitters=8)
+    res_c, secs, lat_c = serving.closed_loop(srv, reqs, submitters=8)
     submit_s = serving.closed_loop.submit_seconds
     capacity = len(reqs) / secs
+    # the server's results equal the sequential plain launches', request by request
+    for k in range(n_seq):
+        assert np.array_equal(res_c[k], seq_out[k]), f"serving: request {k} differs from its plain run"
+    del res_c
     # open loop at 70% of capacity
     rate = 0.7 * capacity
     _, secs_o, lat_o = serving.open_loop(srv, reqs, rate, seed=7)
+    # and at a fixed 1,400 requests/s, the load the 70 % point was before the
+    # query rectangles: latency at an unchanged load
+    fixed_rate = 1400.0
+    _, secs_f, lat_f = serving.open_loop(srv, reqs, fixed_rate, seed=11)
     st = srv.stats()
     srv.close()
     n_q = sum(r.kind == "query" for r in reqs)
@@ -858,9 +878,18 @@ def run_serving(args, torch, capi, workloads, np):
                                "hbm_frac": scan_bytes / (scan_ms / 1e3) / PEAK_HBM},
            "queries": n_q, "all_pairs_jobs": len(reqs) - n_q,
            "capacity_requests_per_s": capacity,
+           "sequential_plain": {"requests": n_seq, "seconds": seq_secs,
+                                "requests_per_s": n_seq / seq_secs,
+                                "note": "one request at a time through hd_reference_distances_dev "
+                                        "(row list) or gather + hd_encode_dev + hd_all_pairs_dev, "
+                                        "result downloaded before the next; results equal the "
+                                        "server's"},
+           "served_vs_sequential": capacity / (n_seq / seq_secs),
            "closed_loop": serving.percentiles(lat_c),
            "offered_rate": rate, "achieved_requests_per_s": len(reqs) / secs_o,
            "open_loop": serving.percentiles(lat_o),
+           "open_loop_fixed_rate": dict(serving.percentiles(lat_f), offered_rate=fixed_rate,
+                                        achieved_requests_per_s=len(reqs) / secs_f),
            "open_loop_by_kind": {
                kind: serving.percentiles([x for x, r in zip(lat_o, reqs) if r.kind == kind])
                for kind in ("query", "all_pairs")},
@@ -872,6 +901,9 @@ def run_serving(args, torch, capi, workloads, np):
            "stats": st}
     log(f"serving: closed loop {secs * 1e3:.0f} ms ({submit_s * 1e3:.0f} ms submitting), "
         f"{rows_scanned * db.shape[1] / secs / 1e12:.2f} TB/s of rows scanned")
+    log(f"serving: capacity {capacity / (n_seq / seq_secs):.2f}x the sequential plain rate")
+    log(f"serving: open loop at {fixed_rate:.0f}/s p50 {np.percentile(lat_f, 50) * 1e3:.2f} ms "
+        f"p99 {np.percentile(lat_f, 99) * 1e3:.2f} ms")
     log(f"serving: capacity {capacity:.0f} req/s; open loop at {rate:.0f}/s "
         f"p50 {res['open_loop']['p50_ms']:.2f} ms p99 {res['open_loop']['p99_ms']:.2f} ms")
     return res
@@ -891,6 +923,9 @@ def main() -> int:
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--serve-rows", type=int, default=1_000_000)
     ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--serve-sequential", type=int, default=400,
+                    help="requests of the serving mix run one at a time through the plain "
+                         "entry points (the sequential baseline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-early-exit-run", action="store_true")
     ap.add_argument("--no-refdist", action="store_true")

@@ -91,6 +91,7 @@ SIGNATURES = {
     "hd_server_release": (C.c_int, [vp, C.c_uint64]),
     "hd_server_stats": (C.c_int, [vp, u64p, u64p, u64p, u64p, u64p]),
     "hd_server_shared_stats": (C.c_int, [vp, u64p, u64p]),
+    "hd_server_rect_stats": (C.c_int, [vp, intp, u64p, u64p]),
     "hd_variant_count": (C.c_int, []),
     "hd_variant_info": (C.c_int, [C.c_int, intp, intp, intp, intp]),
     "hd_default_variant": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, C.c_int,
@@ -410,9 +411,9 @@ class Server:
 
         kind, m, ob = self._meta[rid]
         if kind == "query":
-            out = np.zeros(m, np.uint32)
+            out = np.empty(m, np.uint32)
         else:
-            out = np.zeros(m * (m - 1) // 2, {1: np.uint8, 2: np.uint16, 4: np.uint32}[ob])
+            out = np.empty(m * (m - 1) // 2, {1: np.uint8, 2: np.uint16, 4: np.uint32}[ob])
         lat = C.c_double()
         check(lib().hd_server_result(self._h, rid, out.ctypes.data if out.size else None,
                                      out.nbytes, C.byref(lat)))
@@ -429,4 +430,8 @@ class Server:
         g, q = C.c_uint64(), C.c_uint64()
         check(lib().hd_server_shared_stats(self._h, C.byref(g), C.byref(q)))
         out["shared_groups"], out["shared_queries"] = g.value, q.value
+        en = C.c_int()
+        check(lib().hd_server_rect_stats(self._h, C.byref(en), C.byref(g), C.byref(q)))
+        out["rect_enabled"], out["rect_groups"], out["rect_queries"] = bool(en.value), g.value, \
+            q.value
         return out

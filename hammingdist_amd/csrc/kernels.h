@@ -146,6 +146,19 @@ __device__ __forceinline__ void tile_coords(uint64_t t, uint32_t M, uint32_t &b,
   cb = (uint32_t)(t - (uint64_t)M * r * (r + 1) / 2);
 }
 
+// One 32-site word of the dense 3-op code: row planes (g, c1, c2, c3)
+// against column planes (c1, c2, c3); the popcount of the result plus the
+// row's invalid count minus the column's '-' count is the word's distance
+// (summed over a row).  v_bitop3 truth tables are over src0 = 0xF0, src1 =
+// 0xCC, src2 = 0xAA.
+__device__ __forceinline__ uint32_t dense3_word(uint32_t g, uint32_t c1, uint32_t c2,
+                                                uint32_t c3, uint32_t b1, uint32_t b2,
+                                                uint32_t b3) {
+  uint32_t s = __builtin_amdgcn_bitop3_b32(c1, b1, g, 0xBE); // g | (c1 ^ b1)
+  s = __builtin_amdgcn_bitop3_b32(c2, b2, s, 0x41);
+  return __builtin_amdgcn_bitop3_b32(c3, b3, s, 0x71);
+}
+
 // LDS bytes each wave of a staged-store kernel owns.
 constexpr int kStageBytes = 4096;
 
@@ -365,9 +378,7 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
           // src2 = 0xAA.
           uint32_t s;
           if constexpr (D3) {
-            s = __builtin_amdgcn_bitop3_b32(a[q][1], b[0], a[q][0], 0xBE); // z|(x^y)
-            s = __builtin_amdgcn_bitop3_b32(a[q][2], b[1], s, 0x41);
-            s = __builtin_amdgcn_bitop3_b32(a[q][3], b[2], s, 0x71);
+            s = dense3_word(a[q][0], a[q][1], a[q][2], a[q][3], b[0], b[1], b[2]);
           } else if constexpr (GAP) {
             // codes differ: XOR, then 0xBE = (x ^ y) | z per further bit
             s = a[q][0] ^ b[0];

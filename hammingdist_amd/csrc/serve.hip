@@ -33,7 +33,7 @@ struct QueryDev {
   uint64_t m;
   uint64_t block_begin; // first block of this query in the launch
   int semantics;
-  int pad;
+  int dense3; // the query holds only codes of the dense 3-op code (host-side flag)
 };
 
 // 4 wavefronts x 64 rows: each wavefront loads the query into VGPRs once per
@@ -267,6 +267,16 @@ constexpr uint64_t kSharedMinEntries = 100000;
 constexpr double kSharedMinReuse = 3.0;
 constexpr uint64_t kSharedMaxEntries = 16ull << 20;
 constexpr uint64_t kSharedScanBlocks = 2048; // persistent grid of the shared scan
+// Rates behind the choice between a query rectangle and the group's own
+// scans: database bytes per second of the per-query scans (k_query_batch,
+// 6.2-6.5 TB/s measured) and (row, query) pair-words per second of the
+// rectangle's tile loop (1.17e13 on the all-pairs headline), plus its fixed
+// cost (query encode and launches).
+constexpr double kScanBytesPerSec = 6.0e12;
+constexpr double kRectPairWordsPerSec = 1.1e13;
+constexpr double kRectFixedSec = 30e-6;
+// Bytes of one query-major distance matrix (rect_q_cap x n_db) per stream.
+constexpr uint64_t kRectDistBudget = 1ull << 30;
 
 static const void *shared_scan_kernel(int semantics, uint64_t bpr) {
 #define HD_SQ(T)                                                                          \
@@ -280,6 +290,122 @@ static const void *shared_scan_kernel(int semantics, uint64_t bpr) {
   default: return nullptr;
   }
 #undef HD_SQ
+}
+
+// ---------------------------------------------------------------------------
+// query rectangle: a group of DENSE queries against every database row
+// through the all-pairs tile loop
+// ---------------------------------------------------------------------------
+// A database of only 0, A, C, G, T and '-' codes is also kept as dense 3-op
+// planes (the all-pairs layout: planes[g][k][p][64], then the per-row
+// counts).  A group of up to kRectMaxQ such queries is encoded the same way
+// and one launch computes the distance of every (database row, query) pair
+// of the group: workgroup b holds database rows [256 b, 256 b + 256) in its
+// lanes' VGPRs, wavefront w queries [16 w, 16 w + 16) in SGPRs copied to
+// VGPRs, 3 v_bitop3 + 1 v_bcnt per pair and 32 sites (kernels.h, tile_wave).
+// The distances land in a query-major matrix; a gather then reads each
+// query's row positions out of it.  At 32 queries against 1,000,000 rows of
+// 29,903 sites this is about 2.6 ms, where the queries' own scans of 215,000
+// rows each take 17 ms.
+constexpr uint32_t kRectMaxQ = 64;
+constexpr uint32_t kRectRows = 256;  // database rows per workgroup (R = 4)
+constexpr uint32_t kRectCols = 16;   // queries per wavefront
+// Nibble codes the dense 3-op code encodes: invalid (0), A C G T, '-'.
+constexpr uint32_t kDense3Codes = (1u << 0) | (1u << 1) | (1u << 2) | (1u << 4) | (1u << 8) |
+                                  (1u << 15);
+
+__global__ __launch_bounds__(256) void k_rect_d3(const uint32_t *__restrict__ dbp,
+                                                 uint32_t n_db, uint32_t n_pad_db, uint32_t W,
+                                                 const uint32_t *__restrict__ qp, uint32_t n_q,
+                                                 uint32_t n_pad_q,
+                                                 uint32_t *__restrict__ dist, uint64_t ld) {
+  constexpr int R = kRectRows / kWave, C = kRectCols, PS = 4;
+  constexpr uint32_t OFF = 0x80000000u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint32_t j0 = blockIdx.y * (4 * C) + wave * C;
+  if (j0 >= n_q) return;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t i0 = blockIdx.x * kRectRows;
+  const size_t group = (size_t)W * PS * kWave;
+  const uint32_t *dcorr = dbp + (size_t)W * PS * n_pad_db; // Z totals of the rows
+  const uint32_t *qcorr = qp + (size_t)W * PS * n_pad_q;   // G totals at n_pad_q
+  // acc = OFF + Z_row - G_query + popcounts (the dense 3-op code's per-row
+  // and per-column terms; every partial sum stays within OFF +- 2^24).
+  uint32_t acc[R][C];
+  {
+    uint32_t gj[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) gj[c] = qcorr[n_pad_q + j0 + c];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const uint32_t z = dcorr[min(i0 + kWave * q + lane, n_pad_db - 1)];
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc[q][c] = OFF + z - gj[c];
+    }
+  }
+  using cptr = const __attribute__((address_space(4))) uint32_t *;
+  cptr pb = (cptr)(qp + (size_t)(j0 / kWave) * group + (j0 % kWave));
+  // Rows through one buffer resource at the block's first row group: offsets
+  // below 3 * group * 4 + W * 1024 < 2^31 (W < kBufWords).
+  const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(dbp + (size_t)(i0 / kWave) * group), (short)0, -1, kBufferRsrcWord3);
+  int row_off[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) row_off[q] = (int)((lane + (uint32_t)(q * group)) * 4);
+  for (uint32_t k = 0; k < W; ++k) {
+    uint32_t a[R][PS];
+    const int step_off = (int)(k * (uint32_t)(PS * kWave * 4));
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+#pragma unroll
+      for (int p = 0; p < PS; ++p)
+        a[q][p] = __builtin_amdgcn_raw_buffer_load_b32(rows_rsrc, row_off[q],
+                                                       step_off + p * kWave * 4, 0);
+    uint32_t bw[C][3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int c = 0; c < C; ++c) bw[c][p] = pb[(1 + p) * kWave + c];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const uint32_t b0 = to_vgpr(bw[c][0]), b1 = to_vgpr(bw[c][1]), b2 = to_vgpr(bw[c][2]);
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+        acc[q][c] += __builtin_popcount(dense3_word(a[q][0], a[q][1], a[q][2], a[q][3], b0,
+                                                    b1, b2));
+    }
+    pb += PS * kWave;
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const uint32_t i = i0 + kWave * q + lane;
+    if (i < n_db) {
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+        if (j0 + c < n_q) dist[(uint64_t)(j0 + c) * ld + i] = acc[q][c] - OFF;
+    }
+  }
+}
+
+// Block q copies query q of a group (bpr bytes) into row q of a packed
+// buffer, the input of the group's encode.
+__global__ __launch_bounds__(256) void k_rect_queries(const QueryDev *__restrict__ qs,
+                                                      uint64_t bpr, uint8_t *__restrict__ packed) {
+  const uint2 *src = qs[blockIdx.x].query;
+  uint2 *dst = reinterpret_cast<uint2 *>(packed + (uint64_t)blockIdx.x * bpr);
+  for (uint64_t u = threadIdx.x; u < bpr / 8; u += blockDim.x) dst[u] = src[u];
+}
+
+// out[k] of each query of the group = its distance to database row rows[k]
+// (one thread per row position, the group's blocks as in k_query_batch).
+__global__ __launch_bounds__(256) void k_rect_gather(const QueryDev *__restrict__ qs,
+                                                     uint32_t n_q,
+                                                     const uint32_t *__restrict__ dist,
+                                                     uint64_t ld) {
+  const uint32_t qi = block_query(qs, n_q);
+  const QueryDev &Q = qs[qi];
+  const uint64_t k = (blockIdx.x - Q.block_begin) * kRowsPerBlock + threadIdx.x;
+  if (k < Q.m) Q.out[k] = dist[(uint64_t)qi * ld + Q.rows[k]];
 }
 
 // Copies database rows rows[0..n) into a contiguous packed buffer (one block
@@ -375,6 +501,7 @@ struct hd_server {
     int semantics = HD_SEM_DENSE, out_bytes = 4;
     uint32_t max_dist = 0;
     uint64_t m = 0; // rows of a query, sequences of an all-pairs job
+    bool dense3 = false; // a DENSE query of only 0, A, C, G, T and '-' codes
     std::vector<uint64_t> rows;
     std::vector<uint8_t> query;
     Layout lay;
@@ -417,11 +544,22 @@ struct hd_server {
   // register-resident form).
   struct Scratch {
     DeviceBuffer counts, sums, ent;
+    // query rectangle: the group's packed queries, their planes and the
+    // query-major distance matrix (rect_q_cap x n_db)
+    DeviceBuffer q_packed, q_planes, dist;
   };
   std::vector<std::unique_ptr<Scratch>> scratch;
   uint64_t shared_cap = kSharedMaxEntries; // row positions per shared group
   uint64_t shared_min = kSharedMinEntries;
   double shared_reuse = kSharedMinReuse;
+  // Query rectangle (k_rect_d3): the database's dense 3-op planes, when its
+  // codes allow them; groups of at most rect_q_cap queries.
+  DeviceBuffer db_planes;
+  Geometry g_db, g_q;
+  bool rect = false, rect_force = false;
+  bool shared = false; // shared scans available (the scratch holds their buffers)
+  uint32_t rect_q_cap = 0;
+  uint64_t n_rect_groups = 0, rect_queries = 0;
 
   ~hd_server() {
     {
@@ -463,14 +601,22 @@ struct hd_server {
     return L;
   }
 
-  // Launches every request in ids on stream `slot`; returns the batch.
-  Batch launch(int slot, std::vector<uint64_t> ids) {
+  // Group counts of one launch, added to the server's under its lock.
+  struct LaunchStats {
+    uint64_t rect_groups = 0, rect_queries = 0, shared_groups = 0, shared_queries = 0;
+  };
+
+  // Issues every request of batch b (requests rs, in b.ids order) on the
+  // batch's stream, without the server lock: it reads only what the
+  // requests and the batch already own (their arena blocks, the query
+  // table's blocks b.table_d / b.table_h, the slot's scratch) and the
+  // server's immutable state.
+  void launch(const Batch &b, const std::vector<Request *> &rs, LaunchStats &ls) {
+    const int slot = b.slot;
     hipStream_t s = streams[slot];
-    Batch b;
-    b.slot = slot;
     std::vector<QueryDev> qd;
-    for (uint64_t id : ids) {
-      Request &r = *reqs[id];
+    for (Request *rp : rs) {
+      Request &r = *rp;
       uint8_t *dbase = dev_arena.at(r.d_off), *hbase = host_arena.at(r.h_off);
       if (r.kind == QUERY) {
         HD_HIP(hipMemcpyAsync(dbase, hbase, r.lay.in_bytes, hipMemcpyHostToDevice, s));
@@ -480,6 +626,7 @@ struct hd_server {
         d.out = reinterpret_cast<uint32_t *>(dbase + r.lay.d_out);
         d.m = r.m;
         d.semantics = r.semantics;
+        d.dense3 = r.dense3 ? 1 : 0;
         qd.push_back(d);
       } else {
         HD_HIP(hipMemcpyAsync(dbase + r.lay.d_rows, hbase + r.lay.h_rows, r.lay.in_bytes,
@@ -508,10 +655,50 @@ struct hd_server {
         uint64_t blocks;
         int semantics;
       };
-      std::vector<QueryDev> table, singles;
-      std::vector<Group> groups;
+      std::vector<QueryDev> table, singles, rest;
+      std::vector<Group> groups, rect_groups;
       auto blocks_of = [](const QueryDev &d) { return (d.m + kRowsPerBlock - 1) / kRowsPerBlock; };
-      if (!scratch.empty()) {
+      // Query rectangles first: the batch's dense-code DENSE queries in
+      // arrival order, in groups of at most rect_q_cap, each group taken as
+      // one rectangle over the whole database when that is estimated faster
+      // than the queries' own scans.
+      if (rect) {
+        std::vector<QueryDev> cur;
+        auto flush = [&] {
+          if (cur.empty()) return;
+          uint64_t sum_m = 0;
+          for (const auto &d : cur) sum_m += d.m;
+          const double t_scan = (double)sum_m * (double)bpr / kScanBytesPerSec;
+          const double t_rect =
+              std::max((double)n_db * (double)bpr / kScanBytesPerSec,
+                       (double)cur.size() * (double)n_db * g_db.W / kRectPairWordsPerSec) +
+              (double)sum_m * 16.0 / kScanBytesPerSec + kRectFixedSec;
+          if (rect_force || t_rect < t_scan) {
+            Group g{table.size(), cur.size(), 0, HD_SEM_DENSE};
+            for (auto &d : cur) {
+              d.block_begin = g.blocks;
+              g.blocks += blocks_of(d);
+              table.push_back(d);
+            }
+            rect_groups.push_back(g);
+          } else {
+            rest.insert(rest.end(), cur.begin(), cur.end());
+          }
+          cur.clear();
+        };
+        for (const QueryDev &d : qd) {
+          if (d.semantics == HD_SEM_DENSE && d.dense3) {
+            cur.push_back(d);
+            if (cur.size() == rect_q_cap) flush();
+          } else {
+            rest.push_back(d);
+          }
+        }
+        flush();
+      } else {
+        rest = qd;
+      }
+      if (shared) {
         for (int sem : {HD_SEM_DENSE, HD_SEM_CHAR_X}) {
           std::vector<QueryDev> cur;
           uint64_t cur_m = 0;
@@ -537,7 +724,7 @@ struct hd_server {
             cur.clear();
             cur_m = 0;
           };
-          for (const QueryDev &d : qd) {
+          for (const QueryDev &d : rest) {
             if (d.semantics != sem) continue;
             if (d.m > shared_cap) {
               singles.push_back(d);
@@ -550,7 +737,7 @@ struct hd_server {
           flush();
         }
       } else {
-        singles = qd;
+        singles = rest;
       }
       const size_t single_first = table.size();
       uint64_t blocks = 0;
@@ -560,15 +747,36 @@ struct hd_server {
         table.push_back(d);
       }
       const uint64_t bytes = table.size() * sizeof(QueryDev);
-      b.table_d = dev_arena.alloc(bytes);
-      b.table_h = host_arena.alloc(bytes);
-      if (b.table_d == UINT64_MAX || b.table_h == UINT64_MAX)
-        throw Error(HD_ERR_CAPACITY, "no arena space for the query table");
+      HD_REQUIRE(b.table_d != UINT64_MAX && b.table_h != UINT64_MAX,
+                 "query table not allocated");
       std::memcpy(host_arena.at(b.table_h), table.data(), bytes);
       HD_HIP(hipMemcpyAsync(dev_arena.at(b.table_d), host_arena.at(b.table_h), bytes,
                             hipMemcpyHostToDevice, s));
       const QueryDev *dtab = reinterpret_cast<const QueryDev *>(dev_arena.at(b.table_d));
       const uint8_t *db_ptr = db.get<uint8_t>();
+      for (const Group &g : rect_groups) {
+        Scratch &S = *scratch[slot];
+        const QueryDev *gt = dtab + g.first;
+        const uint32_t n_g = (uint32_t)g.count;
+        HD_REQUIRE(g.blocks < (1ull << 31), "query group too large");
+        hipLaunchKernelGGL(k_rect_queries, dim3(n_g), dim3(256), 0, s, gt, bpr,
+                           S.q_packed.get<uint8_t>());
+        // The same dense 3-op code as the database's planes (its table does
+        // not depend on which of its codes are present).
+        const Geometry gq = geometry(n_g, bpr, HD_SEM_DENSE, db_mask);
+        launch_encode(gq, S.q_packed.get<uint8_t>(), S.q_planes.get<uint32_t>(), s);
+        const dim3 grid((uint32_t)((n_db + kRectRows - 1) / kRectRows),
+                        (n_g + 4 * kRectCols - 1) / (4 * kRectCols));
+        hipLaunchKernelGGL(k_rect_d3, grid, dim3(256), 0, s, db_planes.get<uint32_t>(),
+                           (uint32_t)n_db, (uint32_t)g_db.n_pad, g_db.W,
+                           S.q_planes.get<uint32_t>(), n_g, (uint32_t)gq.n_pad,
+                           S.dist.get<uint32_t>(), n_db);
+        hipLaunchKernelGGL(k_rect_gather, dim3((uint32_t)g.blocks), dim3(256), 0, s, gt, n_g,
+                           S.dist.get<uint32_t>(), n_db);
+        HD_HIP(hipGetLastError());
+        ++ls.rect_groups;
+        ls.rect_queries += g.count;
+      }
       for (const Group &g : groups) {
         Scratch &S = *scratch[slot];
         uint32_t *counts = S.counts.get<uint32_t>();
@@ -595,8 +803,8 @@ struct hd_server {
         const uint32_t grid = (uint32_t)std::min<uint64_t>((n_db + 3) / 4, kSharedScanBlocks);
         HD_HIP(hipLaunchKernel(shared_scan_kernel(g.semantics, bpr), dim3(grid), dim3(256),
                                args, 0, s));
-        ++n_shared_groups;
-        shared_queries += g.count;
+        ++ls.shared_groups;
+        ls.shared_queries += g.count;
       }
       if (!singles.empty()) {
         HD_REQUIRE(blocks < (1ull << 31), "query batch too large");
@@ -608,17 +816,14 @@ struct hd_server {
       }
       HD_HIP(hipGetLastError());
     }
-    for (uint64_t id : ids) {
-      Request &r = *reqs[id];
+    for (Request *rp : rs) {
+      const Request &r = *rp;
       if (r.lay.out_bytes)
         HD_HIP(hipMemcpyAsync(host_arena.at(r.h_off) + r.lay.h_out,
                               dev_arena.at(r.d_off) + r.lay.d_out, r.lay.out_bytes,
                               hipMemcpyDeviceToHost, s));
-      r.state = HD_REQ_RUNNING;
     }
     HD_HIP(hipEventRecord(events[slot], s));
-    b.ids = std::move(ids);
-    return b;
   }
 
   void complete(Batch &b, int state) {
@@ -652,16 +857,20 @@ struct hd_server {
         progressed = true;
       }
       if (progressed) cv_done.notify_all();
-      // 2. admission by footprint, in arrival order
-      while (!pending.empty()) {
-        Request &r = *reqs[pending.front()];
+      // 2. admission by device footprint, in arrival order.  A request
+      // whose inputs found no host memory at submit and find none now does
+      // not hold back the requests behind it whose inputs are staged: they
+      // already own their host memory, and running them is what frees it.
+      for (auto it = pending.begin(); it != pending.end();) {
+        Request &r = *reqs[*it];
         const uint64_t d = dev_arena.alloc(r.lay.dev);
         if (d == UINT64_MAX) break;
         if (r.h_off == UINT64_MAX) { // inputs not staged at submit
           const uint64_t h = host_arena.alloc(r.lay.host);
           if (h == UINT64_MAX) {
             dev_arena.release(d);
-            break;
+            ++it;
+            continue;
           }
           r.h_off = h;
           uint8_t *hb = host_arena.at(h);
@@ -673,7 +882,7 @@ struct hd_server {
         r.d_off = d;
         r.state = HD_REQ_ADMITTED;
         admitted.push_back(r.id);
-        pending.pop_front();
+        it = pending.erase(it);
       }
       // 3. everything admitted so far goes out as one batch on a free stream
       if (!admitted.empty()) {
@@ -684,32 +893,61 @@ struct hd_server {
             break;
           }
         if (slot >= 0) {
-          std::vector<uint64_t> ids;
-          while (!admitted.empty() && ids.size() < max_batch) {
-            ids.push_back(admitted.front());
+          // Taken under the lock: the batch's requests, their state and the
+          // query table's arena blocks; the launch itself runs unlocked, so
+          // submissions and result reads go on meanwhile.
+          Batch b;
+          b.slot = slot;
+          std::vector<Request *> rs;
+          uint64_t n_queries = 0;
+          while (!admitted.empty() && b.ids.size() < max_batch) {
+            const uint64_t id = admitted.front();
             admitted.pop_front();
+            Request *r = reqs[id].get();
+            r->state = HD_REQ_RUNNING;
+            n_queries += r->kind == QUERY ? 1 : 0;
+            b.ids.push_back(id);
+            rs.push_back(r);
           }
-          const std::vector<uint64_t> batch_ids = ids;
-          try {
-            slot_busy[slot] = true;
-            if (!running.empty()) ++n_overlapped;
-            running.push_back(launch(slot, std::move(ids)));
-            ++n_batches;
-            batched_reqs += running.back().ids.size();
-            uint64_t live = 0;
-            for (auto &rb : running) live += rb.ids.size();
-            max_running_reqs = std::max(max_running_reqs, live);
-          } catch (const std::exception &e) {
+          slot_busy[slot] = true;
+          std::string err;
+          if (n_queries) {
+            const uint64_t bytes = n_queries * sizeof(QueryDev);
+            b.table_d = dev_arena.alloc(bytes);
+            b.table_h = host_arena.alloc(bytes);
+            if (b.table_d == UINT64_MAX || b.table_h == UINT64_MAX)
+              err = "no arena space for the query table";
+          }
+          LaunchStats ls;
+          if (err.empty()) {
+            lk.unlock();
+            try {
+              launch(b, rs, ls);
+            } catch (const std::exception &e) {
+              err = e.what();
+            }
+            lk.lock();
+          }
+          if (!err.empty()) {
             // The stream may hold part of the batch: wait for it, then fail
             // every request of the batch.
-            fatal = e.what();
+            fatal = err;
             (void)hipStreamSynchronize(streams[slot]);
-            Batch failed;
-            failed.slot = slot;
-            failed.ids = batch_ids;
-            complete(failed, HD_REQ_FAILED);
+            complete(b, HD_REQ_FAILED);
             cv_done.notify_all();
+            continue;
           }
+          if (!running.empty()) ++n_overlapped;
+          ++n_batches;
+          batched_reqs += b.ids.size();
+          n_rect_groups += ls.rect_groups;
+          rect_queries += ls.rect_queries;
+          n_shared_groups += ls.shared_groups;
+          shared_queries += ls.shared_queries;
+          running.push_back(std::move(b));
+          uint64_t live = 0;
+          for (auto &rb : running) live += rb.ids.size();
+          max_running_reqs = std::max(max_running_reqs, live);
           continue;
         }
       }
@@ -759,16 +997,55 @@ int hd_server_create(const uint8_t *db, uint64_t n_db, uint64_t bytes_per_seq,
       s->shared_min = std::strtoull(e, nullptr, 10);
       s->shared_reuse = 0.0;
     }
-    if (shared_scan_kernel(HD_SEM_DENSE, bytes_per_seq) && n_db < (1ull << 32)) {
-      // Entries for 64 row positions per database row, within [2^16, 2^24].
+    s->shared = shared_scan_kernel(HD_SEM_DENSE, bytes_per_seq) && n_db < (1ull << 32);
+    // Entries for 64 row positions per database row, within [2^16, 2^24].
+    if (s->shared)
       s->shared_cap = std::min<uint64_t>(kSharedMaxEntries,
                                          std::max<uint64_t>(1ull << 16, 64 * n_db));
+    // Query rectangles (HD_SERVER_RECT=0 turns them off; HD_RECT_FORCE=1
+    // takes every group of dense-code queries that way, whatever the cost
+    // estimate says): the database's dense 3-op planes, when its codes allow
+    // the code and the planes, each stream's distance matrix and a margin
+    // fit in free device memory.
+    {
+      const char *off = std::getenv("HD_SERVER_RECT");
+      const char *force = std::getenv("HD_RECT_FORCE");
+      s->rect_force = force && *force == '1';
+      const Geometry g = geometry(n_db, bytes_per_seq, HD_SEM_DENSE, s->db_mask);
+      if (!(off && *off == '0') && (s->db_mask & ~kDense3Codes & 0xFFFFu) == 0 &&
+          is_dense3(g.code) && n_db < (1ull << 31)) {
+        const uint64_t cap = kRectMaxQ * n_db * 4 <= kRectDistBudget ? kRectMaxQ : kRectCols;
+        const Geometry gq = geometry(kRectMaxQ, bytes_per_seq, HD_SEM_DENSE, s->db_mask);
+        const uint64_t per_stream = cap * n_db * 4 + kRectMaxQ * bytes_per_seq +
+                                    gq.total_words() * 4;
+        const uint64_t need = g.total_words() * 4 + (uint64_t)streams * per_stream;
+        size_t free_b = 0, total_b = 0;
+        HD_HIP(hipMemGetInfo(&free_b, &total_b));
+        if (need + (2ull << 30) <= free_b) {
+          s->db_planes.alloc(g.total_words() * 4);
+          launch_encode(g, s->db.get<uint8_t>(), s->db_planes.get<uint32_t>(), nullptr);
+          HD_HIP(hipDeviceSynchronize());
+          s->g_db = g;
+          s->g_q = gq;
+          s->rect_q_cap = (uint32_t)cap;
+          s->rect = true;
+        }
+      }
+    }
+    if (s->shared || s->rect) {
       const uint64_t nb = (n_db + kScanBlock - 1) / kScanBlock;
       for (int k = 0; k < streams; ++k) {
         auto sc = std::make_unique<hd_server::Scratch>();
-        sc->counts.alloc(n_db * 4);
-        sc->sums.alloc(nb * 4);
-        sc->ent.alloc(s->shared_cap * sizeof(SharedEntry));
+        if (s->shared) {
+          sc->counts.alloc(n_db * 4);
+          sc->sums.alloc(nb * 4);
+          sc->ent.alloc(s->shared_cap * sizeof(SharedEntry));
+        }
+        if (s->rect) {
+          sc->q_packed.alloc(kRectMaxQ * bytes_per_seq);
+          sc->q_planes.alloc(s->g_q.total_words() * 4);
+          sc->dist.alloc((uint64_t)s->rect_q_cap * n_db * 4);
+        }
         s->scratch.push_back(std::move(sc));
       }
     }
@@ -800,6 +1077,14 @@ static int submit(hd_server *s, std::unique_ptr<hd_server::Request> r, const uin
   return guarded([&] {
     HD_REQUIRE(s && id, "null server or id");
     if (r->kind == hd_server::ALLPAIRS) r->g = geometry(r->m, s->bpr, r->semantics, s->db_mask);
+    if (r->kind == hd_server::QUERY && r->semantics == HD_SEM_DENSE && s->rect) {
+      // The codes of the query: a rectangle takes it when they are all
+      // codes of the dense 3-op code.
+      uint32_t mask = 0;
+      for (uint64_t b = 0; b < s->bpr; ++b)
+        mask |= (1u << (query[b] & 15u)) | (1u << (query[b] >> 4));
+      r->dense3 = (mask & ~kDense3Codes) == 0;
+    }
     r->lay = s->layout(*r);
     if (r->lay.dev > s->dev_arena.size() || r->lay.host > s->host_arena.size())
       throw Error(HD_ERR_CAPACITY, "request larger than the server's memory budget");
@@ -898,17 +1183,23 @@ int hd_server_wait(hd_server *s, uint64_t id, double timeout_s, int *state) {
 int hd_server_result(hd_server *s, uint64_t id, void *out, uint64_t bytes, double *latency_s) {
   return guarded([&] {
     HD_REQUIRE(s, "null server");
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(id);
-    HD_REQUIRE(it != s->reqs.end(), "unknown request");
-    hd_server::Request &r = *it->second;
-    HD_REQUIRE(r.state == HD_REQ_DONE, "request not done");
-    HD_REQUIRE(bytes >= r.lay.out_bytes, "result buffer too small");
-    if (r.lay.out_bytes) {
-      HD_REQUIRE(out, "null buffer");
-      std::memcpy(out, s->host_arena.at(r.h_off) + r.lay.h_out, r.lay.out_bytes);
+    const uint8_t *src = nullptr;
+    uint64_t n = 0;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      auto it = s->reqs.find(id);
+      HD_REQUIRE(it != s->reqs.end(), "unknown request");
+      const hd_server::Request &r = *it->second;
+      HD_REQUIRE(r.state == HD_REQ_DONE, "request not done");
+      HD_REQUIRE(bytes >= r.lay.out_bytes, "result buffer too small");
+      HD_REQUIRE(out || r.lay.out_bytes == 0, "null buffer");
+      src = s->host_arena.at(r.h_off) + r.lay.h_out;
+      n = r.lay.out_bytes;
+      if (latency_s) *latency_s = std::chrono::duration<double>(r.t_done - r.t_submit).count();
     }
-    if (latency_s) *latency_s = std::chrono::duration<double>(r.t_done - r.t_submit).count();
+    // A done request's result stays in its host block until the request is
+    // released, which the caller does not do while it reads the result.
+    if (n) std::memcpy(out, src, n);
   });
 }
 
@@ -947,6 +1238,16 @@ int hd_server_shared_stats(hd_server *s, uint64_t *groups, uint64_t *queries) {
     std::lock_guard<std::mutex> lk(s->mu);
     if (groups) *groups = s->n_shared_groups;
     if (queries) *queries = s->shared_queries;
+  });
+}
+
+int hd_server_rect_stats(hd_server *s, int *enabled, uint64_t *groups, uint64_t *queries) {
+  return guarded([&] {
+    HD_REQUIRE(s, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (enabled) *enabled = s->rect ? 1 : 0;
+    if (groups) *groups = s->n_rect_groups;
+    if (queries) *queries = s->rect_queries;
   });
 }
 

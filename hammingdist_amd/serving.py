@@ -85,23 +85,74 @@ def run_alone(server: "capi.Server", reqs: list[Request]):
     return out
 
 
+def run_sequential_plain(capi_mod, torch, d_db, n_db: int, bpr: int, reqs: list[Request],
+                         stream: int):
+    """The requests one at a time through the plain device entry points, as
+    a client without the server would run them: a query is one
+    hd_reference_distances_dev launch over its row list; an all-pairs job
+    gathers its rows, takes their nibble mask, encodes and runs
+    hd_all_pairs_dev (early exit on, default tile shape).  Each result is
+    downloaded before the next request starts.  d_db: the packed database
+    on the device (n_db x bpr bytes)."""
+    dev = d_db.device
+    rows2d = d_db.view(n_db, bpr)
+    out = []
+    for r in reqs:
+        if r.kind == "query":
+            d_q = torch.from_numpy(np.ascontiguousarray(r.query)).to(dev)
+            d_rows = torch.from_numpy(r.rows.astype(np.int64)).to(dev)
+            d_out = torch.empty(r.rows.size, dtype=torch.int32, device=dev)
+            capi_mod.reference_distances_dev(d_q, d_db, n_db, bpr, d_rows, r.rows.size,
+                                             capi_mod.HD_SEM_DENSE, d_out, stream)
+            out.append(d_out.cpu().numpy().view(np.uint32))
+        else:
+            n = r.rows.size
+            idx = torch.from_numpy(r.rows.astype(np.int64)).to(dev)
+            d_packed = rows2d.index_select(0, idx).reshape(-1)
+            sem = capi_mod.HD_SEM_DENSE
+            mask = capi_mod.nibble_mask_dev(d_packed, n, bpr, stream)
+            _, _, _, total = capi_mod.planes_layout(n, bpr, sem, mask)
+            d_planes = torch.empty(total, dtype=torch.int32, device=dev)
+            capi_mod.encode_dev(d_packed, n, bpr, sem, mask, d_planes, stream)
+            cells = n * (n - 1) // 2
+            dt = {1: torch.uint8, 2: torch.int16, 4: torch.int32}[r.out_bytes]
+            d_out = torch.empty(cells, dtype=dt, device=dev)
+            if n >= 2:
+                capi_mod.all_pairs_dev(d_planes, n, bpr, sem, mask, r.out_bytes, r.max_dist,
+                                       True, -1, d_out, stream)
+            host = d_out.cpu().numpy()
+            out.append(host.view({1: np.uint8, 2: np.uint16, 4: np.uint32}[r.out_bytes]))
+    return out
+
+
 def closed_loop(server: "capi.Server", reqs: list[Request], submitters: int = 4):
     """All requests at once, submitted by `submitters` client threads (each
     submit stages its inputs in the caller's thread, so several clients
-    submit in parallel); returns (results, seconds, latencies)."""
+    submit in parallel); the results are read in submission order after
+    that.  Returns (results, seconds, latencies); seconds runs from the first
+    submission until the last request is done, i.e. its result is in the
+    server's pinned host memory: each request's done time is the time its
+    submit call returned plus the latency the server measured from inside
+    that call (an upper bound).  Copying the results out is the client's
+    work and not counted."""
     from concurrent.futures import ThreadPoolExecutor
+
+    def one(r):
+        rid = submit(server, r)
+        return rid, time.perf_counter()
 
     t = time.perf_counter()
     with ThreadPoolExecutor(max_workers=submitters) as ex:
-        ids = list(ex.map(lambda r: submit(server, r), reqs))
+        subs = list(ex.map(one, reqs))
     closed_loop.submit_seconds = time.perf_counter() - t
-    res, lat = [], []
-    for rid in ids:
+    res, lat, done = [], [], 0.0
+    for rid, t_ret in subs:
         assert server.wait(rid) == capi.HD_REQ_DONE
         a, l = server.result(rid)
         res.append(a)
         lat.append(l)
-    return res, time.perf_counter() - t, lat
+        done = max(done, t_ret + l)
+    return res, done - t, lat
 
 
 def open_loop(server: "capi.Server", reqs: list[Request], rate: float, seed: int = 0):

@@ -274,7 +274,8 @@ int hd_server_submit_all_pairs(hd_server *server, const uint64_t *rows,
  * (timeout_s < 0: no limit); *state receives HD_REQ_*.                     */
 int hd_server_wait(hd_server *server, uint64_t request, double timeout_s,
                    int *state);
-/* Copies a done request's result; *latency_s = completion - submission.   */
+/* Copies a done request's result; *latency_s = completion - submission.
+ * Calls for one request must not overlap hd_server_release of it.         */
 int hd_server_result(hd_server *server, uint64_t request, void *out,
                      uint64_t bytes, double *latency_s);
 int hd_server_release(hd_server *server, uint64_t request);
@@ -284,6 +285,14 @@ int hd_server_stats(hd_server *server, uint64_t *done, uint64_t *batches,
 /* Query groups the server ran as shared scans (one read of a database row for
  * every query of the group that holds it), and the queries in them.         */
 int hd_server_shared_stats(hd_server *server, uint64_t *groups, uint64_t *queries);
+/* Query rectangles: enabled = 1 when the server keeps its database also as
+ * dense 3-op planes (a database of only 0, A, C, G, T and '-' codes, with
+ * room in device memory); a rectangle computes a group of up to 64 DENSE
+ * queries of those codes against every database row in one all-pairs tile
+ * launch and then gathers each query's rows.  groups, queries: the groups
+ * run that way and the queries in them.                                     */
+int hd_server_rect_stats(hd_server *server, int *enabled, uint64_t *groups,
+                         uint64_t *queries);
 
 /* Number of all-pairs tile shapes compiled in, and a description of one.    */
 int hd_variant_count(void);

@@ -71,15 +71,20 @@ def test_oversized_request_is_rejected(db):
     srv.close()
 
 
-@pytest.mark.parametrize("min_entries", [None, "1"])
-def test_shared_scan_matches_oracle(db, min_entries, monkeypatch):
+@pytest.mark.parametrize("mode", ["default", "shared", "rect"])
+def test_shared_scan_matches_oracle(db, mode, monkeypatch):
     # Queries of overlapping row sets (rows repeated within a query too),
     # DENSE and CHAR_X, submitted together so that they batch: groups of a
     # semantics read each database row once for all of their queries
-    # (HD_SHARED_MIN_ENTRIES=1 takes every group of two or more that way).
+    # (shared: HD_SHARED_MIN_ENTRIES=1 takes every group of two or more that
+    # way, with the query rectangles off), or the DENSE ones run as one query
+    # rectangle over the whole database (rect: HD_RECT_FORCE=1).
     packed, L = db
-    if min_entries:
-        monkeypatch.setenv("HD_SHARED_MIN_ENTRIES", min_entries)
+    if mode == "shared":
+        monkeypatch.setenv("HD_SHARED_MIN_ENTRIES", "1")
+        monkeypatch.setenv("HD_SERVER_RECT", "0")
+    elif mode == "rect":
+        monkeypatch.setenv("HD_RECT_FORCE", "1")
     rng = np.random.default_rng(17)
     n_db = packed.shape[0]
     hot = rng.choice(n_db, size=300, replace=False)
@@ -105,6 +110,81 @@ def test_shared_scan_matches_oracle(db, min_entries, monkeypatch):
             want = [O.distance_char(text[src], text[int(r)]) for r in rows]
         np.testing.assert_array_equal(got, np.array(want, np.uint32))
     st = srv.stats()
-    if min_entries:
+    if mode == "shared":
         assert st["shared_groups"] >= 1 and st["shared_queries"] >= 2
+        assert not st["rect_enabled"] and st["rect_groups"] == 0
+    elif mode == "rect":
+        assert st["rect_enabled"] and st["rect_groups"] >= 1 and st["rect_queries"] >= 2
+    srv.close()
+
+
+def test_query_rectangles_uneven_groups(db, monkeypatch):
+    # 70 DENSE queries of the dense 3-op codes in one batch (a rectangle of
+    # 64 and one of 6: partial wavefronts and a partial last row block of the
+    # 4,000-row database), beside DENSE queries holding X (not of the code:
+    # their own scans) and CHAR_X queries; every result against the oracle,
+    # and each the same as the request alone.
+    packed, L = db
+    monkeypatch.setenv("HD_RECT_FORCE", "1")
+    rng = np.random.default_rng(23)
+    n_db, bps = packed.shape
+    reqs = []
+    srcs = []
+    for k in range(84):
+        m = int(rng.integers(1, 2 * n_db))  # rows repeat within a query
+        rows = rng.integers(0, n_db, m).astype(np.uint64)
+        src = int(rng.integers(0, n_db))
+        q = packed[src].copy()
+        sem = capi.HD_SEM_DENSE
+        if k % 12 == 5:
+            q[int(rng.integers(0, bps))] = 0x33  # X at two sites
+        elif k % 12 == 11:
+            sem = capi.HD_SEM_CHAR_X
+        reqs.append((q, rows, sem))
+        srcs.append(src)
+    srv = capi.Server(packed, device_bytes=2 << 30, host_bytes=512 << 20, streams=1,
+                      max_batch=128)
+    ids = [srv.submit_query(q, rows, sem) for q, rows, sem in reqs]
+    got = []
+    for rid in ids:
+        assert srv.wait(rid) == capi.HD_REQ_DONE
+        got.append(srv.result(rid)[0])
+    st = srv.stats()
+    assert st["rect_enabled"] and st["rect_groups"] >= 2 and st["rect_queries"] == 70
+    for (q, rows, sem), g in zip(reqs[:20], got[:20]):
+        alone = serving.run_alone(srv, [serving.Request("query", rows, q)])[0] \
+            if sem == capi.HD_SEM_DENSE else g
+        np.testing.assert_array_equal(g, alone)
+    chars = workloads.generate_chars(workloads.large(n=n_db, L=L, seed=77))
+    text = [bytes(c).decode() for c in chars]
+    uniq_cache = {}
+    for (q, rows, sem), src, g in zip(reqs, srcs, got):
+        if sem == capi.HD_SEM_DENSE:
+            key = q.tobytes()
+            if key not in uniq_cache:
+                uniq_cache[key] = np.array([O.distance_dense(q, packed[r]) for r in range(n_db)],
+                                           np.uint32)
+            np.testing.assert_array_equal(g, uniq_cache[key][rows.astype(np.int64)])
+        else:
+            want = [O.distance_char(text[src], text[int(r)]) for r in rows[:200]]
+            np.testing.assert_array_equal(g[:200], np.array(want, np.uint32))
+    srv.close()
+
+
+def test_query_rectangles_off_for_codes_outside_the_dense_code(db, monkeypatch):
+    # A database holding X (nibble 3) keeps the per-query and shared scans.
+    packed, L = db
+    monkeypatch.setenv("HD_RECT_FORCE", "1")
+    with_x = packed.copy()
+    with_x[5, 3] = 0x33
+    srv = capi.Server(with_x, device_bytes=1 << 30, host_bytes=256 << 20, streams=1)
+    rows = np.arange(with_x.shape[0], dtype=np.uint64)
+    ids = [srv.submit_query(with_x[k], rows) for k in (0, 5, 9)]
+    for k, rid in zip((0, 5, 9), ids):
+        assert srv.wait(rid) == capi.HD_REQ_DONE
+        got, _ = srv.result(rid)
+        want = [O.distance_dense(with_x[k], with_x[r]) for r in range(with_x.shape[0])]
+        np.testing.assert_array_equal(got, np.array(want, np.uint32))
+    st = srv.stats()
+    assert not st["rect_enabled"] and st["rect_groups"] == 0
     srv.close()
