@@ -300,32 +300,39 @@ static const void *shared_scan_kernel(int semantics, uint64_t bpr) {
 // planes (the all-pairs layout: planes[g][k][p][64], then the per-row
 // counts).  A group of up to kRectMaxQ such queries is encoded the same way
 // and one launch computes the distance of every (database row, query) pair
-// of the group: workgroup b holds database rows [256 b, 256 b + 256) in its
-// lanes' VGPRs, wavefront w queries [16 w, 16 w + 16) in SGPRs copied to
-// VGPRs, 3 v_bitop3 + 1 v_bcnt per pair and 32 sites (kernels.h, tile_wave).
+// of the group: workgroup b holds database rows [64 R b, 64 R (b + 1)) in
+// its lanes' VGPRs (R rows per lane), wavefront w queries [16 w, 16 w + 16)
+// in SGPRs copied to VGPRs, 3 v_bitop3 + 1 v_bcnt per pair and 32 sites
+// (kernels.h, tile_wave).
 // The distances land in a query-major matrix; a gather then reads each
 // query's row positions out of it.  At 32 queries against 1,000,000 rows of
 // 29,903 sites this is about 2.6 ms, where the queries' own scans of 215,000
 // rows each take 17 ms.
 constexpr uint32_t kRectMaxQ = 64;
-constexpr uint32_t kRectRows = 256;  // database rows per workgroup (R = 4)
-constexpr uint32_t kRectCols = 16;   // queries per wavefront
+constexpr uint32_t kRectCols = 16; // queries per wavefront
+// Groups of at most this many queries take 2 database rows per lane (128-row
+// workgroups, 48 VGPRs, up to 8 wavefronts per SIMD): with few queries the
+// tile loop waits on its row loads, and twice the wavefronts keep more of
+// them in flight; larger groups take 4 rows per lane (256-row workgroups,
+// 92 VGPRs), a copied query word serving 4 rows.
+constexpr uint32_t kRectSmallQ = 32;
 // Nibble codes the dense 3-op code encodes: invalid (0), A C G T, '-'.
 constexpr uint32_t kDense3Codes = (1u << 0) | (1u << 1) | (1u << 2) | (1u << 4) | (1u << 8) |
                                   (1u << 15);
 
+template <int R>
 __global__ __launch_bounds__(256) void k_rect_d3(const uint32_t *__restrict__ dbp,
                                                  uint32_t n_db, uint32_t n_pad_db, uint32_t W,
                                                  const uint32_t *__restrict__ qp, uint32_t n_q,
                                                  uint32_t n_pad_q,
                                                  uint32_t *__restrict__ dist, uint64_t ld) {
-  constexpr int R = kRectRows / kWave, C = kRectCols, PS = 4;
+  constexpr int C = kRectCols, PS = 4;
   constexpr uint32_t OFF = 0x80000000u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const uint32_t j0 = blockIdx.y * (4 * C) + wave * C;
   if (j0 >= n_q) return;
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint32_t i0 = blockIdx.x * kRectRows;
+  const uint32_t i0 = blockIdx.x * (kWave * R);
   const size_t group = (size_t)W * PS * kWave;
   const uint32_t *dcorr = dbp + (size_t)W * PS * n_pad_db; // Z totals of the rows
   const uint32_t *qcorr = qp + (size_t)W * PS * n_pad_q;   // G totals at n_pad_q
@@ -346,7 +353,7 @@ __global__ __launch_bounds__(256) void k_rect_d3(const uint32_t *__restrict__ db
   using cptr = const __attribute__((address_space(4))) uint32_t *;
   cptr pb = (cptr)(qp + (size_t)(j0 / kWave) * group + (j0 % kWave));
   // Rows through one buffer resource at the block's first row group: offsets
-  // below 3 * group * 4 + W * 1024 < 2^31 (W < kBufWords).
+  // below (R - 1) * group * 4 + W * 1024 < 2^31 (W < kBufWords).
   const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
       (void *)(dbp + (size_t)(i0 / kWave) * group), (short)0, -1, kBufferRsrcWord3);
   int row_off[R];
@@ -765,11 +772,13 @@ struct hd_server {
         // not depend on which of its codes are present).
         const Geometry gq = geometry(n_g, bpr, HD_SEM_DENSE, db_mask);
         launch_encode(gq, S.q_packed.get<uint8_t>(), S.q_planes.get<uint32_t>(), s);
-        const dim3 grid((uint32_t)((n_db + kRectRows - 1) / kRectRows),
+        const bool small = n_g <= kRectSmallQ;
+        const uint64_t rows_per_block = small ? 128 : 256;
+        const dim3 grid((uint32_t)((n_db + rows_per_block - 1) / rows_per_block),
                         (n_g + 4 * kRectCols - 1) / (4 * kRectCols));
-        hipLaunchKernelGGL(k_rect_d3, grid, dim3(256), 0, s, db_planes.get<uint32_t>(),
-                           (uint32_t)n_db, (uint32_t)g_db.n_pad, g_db.W,
-                           S.q_planes.get<uint32_t>(), n_g, (uint32_t)gq.n_pad,
+        hipLaunchKernelGGL(small ? k_rect_d3<2> : k_rect_d3<4>, grid, dim3(256), 0, s,
+                           db_planes.get<uint32_t>(), (uint32_t)n_db, (uint32_t)g_db.n_pad,
+                           g_db.W, S.q_planes.get<uint32_t>(), n_g, (uint32_t)gq.n_pad,
                            S.dist.get<uint32_t>(), n_db);
         hipLaunchKernelGGL(k_rect_gather, dim3((uint32_t)g.blocks), dim3(256), 0, s, gt, n_g,
                            S.dist.get<uint32_t>(), n_db);
