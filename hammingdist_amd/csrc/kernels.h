@@ -235,6 +235,42 @@ __device__ __forceinline__ void tile_store_staged(const uint32_t (&v)[R][C], uin
   }
 }
 
+// One wave's share of filling the cells of rows [i0, i1), one contiguous run
+// of the triangle (cells i0 (i0 - 1) / 2 .. i1 (i1 - 1) / 2 - 1, less
+// cell_offset), with max_dist as SB-byte cells: share `part` of `parts` of
+// the run's 16-byte units, streaming stores, and with part 0 the cells before
+// the first unit and after the last.
+template <int SB>
+__device__ __forceinline__ void fill_rows_share(void *out, uint32_t i0, uint32_t i1,
+                                                uint64_t cell_offset, uint32_t max_dist,
+                                                uint32_t part, uint32_t parts) {
+  using T = typename std::conditional<
+      SB == 1, uint8_t, typename std::conditional<SB == 2, uint16_t, uint32_t>::type>::type;
+  using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
+  constexpr uint32_t EPU = 16 / SB;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  T *const dst = static_cast<T *>(out);
+  const uint64_t e0 = (uint64_t)i0 * (i0 ? i0 - 1 : 0) / 2 - cell_offset;
+  const uint64_t e1 = (uint64_t)i1 * (i1 ? i1 - 1 : 0) / 2 - cell_offset;
+  const uint64_t mis = ((uintptr_t)dst / SB + e0) % EPU;
+  const uint64_t a = min(e1, e0 + (mis ? EPU - mis : 0));
+  const uint64_t units = (e1 - a) / EPU;
+  const uint64_t tail = a + units * EPU;
+  const T v = (T)max_dist;
+  uint32_t w = v;
+  if (SB == 1) w *= 0x01010101u;
+  else if (SB == 2) w |= w << 16;
+  if (part == 0) {
+    if (e0 + lane < a) dst[e0 + lane] = v;
+    if (tail + lane < e1) dst[tail + lane] = v;
+  }
+  const u32x4 q = {w, w, w, w};
+  u32x4 *const u = reinterpret_cast<u32x4 *>(dst + a);
+  const uint64_t u0 = units * part / parts, u1 = units * (part + 1) / parts;
+  // streaming stores: nothing reads the cells back in this launch
+  for (uint64_t k = u0 + lane; k < u1; k += kWave) __builtin_nontemporal_store(q, u + k);
+}
+
 // One wave's share of a tile: rows i0 + 64 q + lane (q < R) against columns
 // j0 .. j0 + C - 1, stored into the triangle at cell i(i-1)/2 + j - cell_offset
 // as out_bytes-wide cells (1, 2 or 4; a constant where the caller's is).
@@ -243,7 +279,11 @@ __device__ __forceinline__ void tile_store_staged(const uint32_t (&v)[R][C], uin
 // EE = false compiles the loop without early-exit votes (check_every is then
 // ignored).  SB = out_bytes (1, 2 or 4) stores the cells through `stage`,
 // kStageBytes of LDS owned by this wave (tile_store_staged); SB = 0 stores
-// them straight from the accumulators.
+// them straight from the accumulators.  fill_parts > 0 (kDense3 with votes,
+// SB != 0, every tile of the row block in the launch): when every row of the
+// tile's row block holds at least max_dist invalid sites, the wave writes
+// share fill_part of fill_parts of the block's cells instead of its own
+// (fill_rows_share).
 template <int P, int R, int C, bool EE = true, int SB = 0>
 __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, uint32_t n,
                                           uint32_t W, uint32_t i0, uint32_t j0,
@@ -251,7 +291,8 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
                                           uint64_t cell_offset, uint32_t max_dist,
                                           uint32_t check_every,
                                           const uint32_t *__restrict__ corr,
-                                          uint32_t n_pad, uint8_t *stage = nullptr) {
+                                          uint32_t n_pad, uint8_t *stage = nullptr,
+                                          uint32_t fill_part = 0, uint32_t fill_parts = 0) {
   constexpr bool D3 = P == kDense3;
   constexpr bool GAP = P == kGap2 || P == kGap3;
   constexpr bool CLS = P == kClass3;
@@ -276,6 +317,33 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   const uint32_t md = OFFSET ? min(max_dist, 0x10000000u) : max_dist;
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t i_end = min(i0 + (uint32_t)(kWave * R), n);
+
+  // kDense3 with votes: an invalid site counts against every site of every
+  // other row ((0 & b) == 0), so d(i, j) >= max(Z_i, Z_j) with Z the row's
+  // invalid sites.  A pair whose bound reaches the cap is settled before the
+  // loop: it starts at kD3Idle like a pair outside the triangle and stores
+  // max_dist (the bound applies only when max_dist <= Z < 2^23, so md is
+  // max_dist).  A wave whose pairs are all settled skips the loop.  When
+  // every row of the row block is settled, all the block's cells are
+  // max_dist and the wave fills its share of them (every wave of the block's
+  // tiles holds the same rows and takes the same branch).
+  const bool bound = D3 && EE && check_every < W;
+  uint32_t zi[R];
+  if (bound) {
+    bool rows_settled = true;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const uint32_t i = i0 + kWave * q + lane;
+      zi[q] = corr[min(i, n_pad - 1)];
+      rows_settled = rows_settled && (i >= n || zi[q] >= md);
+    }
+    if constexpr (SB != 0) {
+      if (fill_parts && __all(rows_settled)) {
+        fill_rows_share<SB>(out, i0, i_end, cell_offset, max_dist, fill_part, fill_parts);
+        return;
+      }
+    }
+  }
   if (j0 + 1 >= i_end) return; // no pair j < i inside this wave's block
 
   // Pairs outside the triangle start saturated so that the early-exit vote
@@ -299,13 +367,27 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
       for (int c = 0; c < C; ++c) acc[q][c] += zi - gj[c];
     }
   };
+  bool settled = false;
   if constexpr (OFFSET && EE) {
+    uint32_t zj[C];
+    if (bound) {
+      uint32_t vzero;
+      __asm__ volatile("v_mov_b32 %0, 0" : "=v"(vzero));
+#pragma unroll
+      for (int c = 0; c < C; ++c) zj[c] = corr[j0 + c + vzero];
+    }
+    uint32_t lo = kD3Idle;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const uint32_t i = i0 + kWave * q + lane;
 #pragma unroll
-      for (int c = 0; c < C; ++c) acc[q][c] = (j0 + c < i && i < n) ? OFF : kD3Idle;
+      for (int c = 0; c < C; ++c) {
+        const bool open = j0 + c < i && i < n && !(bound && max(zi[q], zj[c]) >= md);
+        acc[q][c] = open ? OFF : kD3Idle;
+        lo = min(lo, acc[q][c]);
+      }
     }
+    if (bound) settled = __all(lo >= OFF + md);
   } else if constexpr (OFFSET) {
     // No votes: the whole-row counts go in at the start.
     uint32_t gj[C];
@@ -346,7 +428,7 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   int row_off[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) row_off[q] = (int)((lane + (uint32_t)(q * group)) * 4);
-  uint32_t k = 0, chunk = 0;
+  uint32_t k = settled ? W : 0, chunk = 0;
   while (k < W) {
     const uint32_t k_end = EE ? min(W, k + check_every) : W;
     for (; k < k_end; ++k) {
@@ -526,9 +608,12 @@ __device__ __forceinline__ void allpairs_tile(
     // go through LDS.  Tiles that run every word store straight from the
     // registers (measured as fast, with no LDS held).
     __shared__ __attribute__((aligned(16))) uint8_t stage[NW][kStageBytes];
+    // Every tile of a row block is in the launch (launches cover whole row
+    // blocks), so the block's M (b + 1) tiles x NW waves share its fill.
     tile_wave<P, R, C, EE, (int)sizeof(OutT)>(planes, n, W, b * TI, cb * TJ + wave * C, out,
                                               sizeof(OutT), cell_offset, max_dist,
-                                              check_every, corr, n_pad, stage[wave]);
+                                              check_every, corr, n_pad, stage[wave],
+                                              cb * NW + wave, (TI / TJ) * (b + 1) * NW);
   } else {
     tile_wave<P, R, C, EE>(planes, n, W, b * TI, cb * TJ + wave * C, out, sizeof(OutT),
                            cell_offset, max_dist, check_every, corr, n_pad);

@@ -162,6 +162,93 @@ def test_early_exit_identical_on_saturating_data():
         np.testing.assert_array_equal(got, want)
 
 
+def invalid_count_rows(rng, n, L, blocks):
+    """Rows of one root with few substitutions and '-' runs, and exactly z
+    invalid sites ('N' or '?', scattered or in a run) per row, z taken per
+    block of rows from `blocks` ((row count, z choices) pairs, cycled)."""
+    root = rng.choice(list("ACGT"), size=L)
+    seqs, k = [], 0
+    while len(seqs) < n:
+        count, zs = blocks[k % len(blocks)]
+        k += 1
+        for _ in range(min(count, n - len(seqs))):
+            s = root.copy()
+            s[rng.integers(0, L, size=rng.integers(0, 3))] = rng.choice(list("ACGT"))
+            a = int(rng.integers(0, L))
+            s[a:a + int(rng.integers(0, 200))] = "-"
+            z = int(rng.choice(zs))
+            if z and rng.random() < 0.3:
+                b = int(rng.integers(0, L - z + 1))
+                s[b:b + z] = "N"
+            elif z:
+                s[rng.choice(L, size=z, replace=False)] = rng.choice(list("N?"))
+            seqs.append("".join(s))
+    return seqs
+
+
+@pytest.mark.parametrize("variant", range(5))
+def test_invalid_site_bound_settles_exactly(variant):
+    # With early-exit votes the dense 3-op kernel settles pairs whose bound
+    # max(Z_i, Z_j) (invalid sites of either row) reaches the cap before its
+    # loop.  Row blocks with every row's Z >= the cap, with Z just under it,
+    # with no invalid site and with mixed counts put waves of every tile
+    # shape fully settled, partly settled and open; a pair of rows under the
+    # cap whose invalid sites share positions has a distance below Z_i + Z_j.
+    rng = np.random.default_rng(4242)
+    n, L = 1300, 2500
+    blocks = [(300, [2, 3, 40]), (70, [0]), (200, [1]), (130, [0, 1, 2, 3, 4, 5]),
+              (256, [6, 9]), (64, [0, 2])]
+    seqs = invalid_count_rows(rng, n, L, blocks)
+    packed = O.pack(O.as_chars(seqs))
+    for out_bytes, md in [(1, 1), (1, 2), (1, 3), (2, 5), (4, 7), (1, 40)]:
+        want = O.all_pairs_dense(packed, out_bytes, md)
+        got = run_dev(packed, capi.HD_SEM_DENSE, out_bytes, md, True, variant)
+        np.testing.assert_array_equal(got, want, err_msg=f"v={variant} md={md}")
+
+
+@pytest.mark.parametrize("variant", [0, 4])
+def test_invalid_site_fill_at_shifted_outputs_and_in_bands(variant):
+    # Row blocks whose rows all reach the cap are filled with 16-byte stores
+    # from the first 16-byte boundary of the output on: outputs that start 1
+    # cell past an aligned address, and row bands (band-relative cells), move
+    # every block's run against that boundary.  The guard cells around the
+    # output keep their pattern.
+    rng = np.random.default_rng(77)
+    n, L = 1400, 1500
+    blocks = [(512, [3, 8]), (256, [0, 1]), (600, [2, 30])]
+    packed = O.pack(O.as_chars(invalid_count_rows(rng, n, L, blocks)))
+    stream = torch.cuda.current_stream().cuda_stream
+    d_packed = torch.from_numpy(np.ascontiguousarray(packed)).cuda()
+    mask = capi.nibble_mask_dev(d_packed, n, packed.shape[1], stream)
+    _, _, _, total = capi.planes_layout(n, packed.shape[1], capi.HD_SEM_DENSE, mask)
+    d_planes = torch.empty(total, dtype=torch.int32, device="cuda")
+    capi.encode_dev(d_packed, n, packed.shape[1], capi.HD_SEM_DENSE, mask, d_planes, stream)
+    cells = n * (n - 1) // 2
+    TI = 64 * capi.variants()[variant]["rows_per_lane"]
+    for out_bytes, md in [(1, 2), (2, 3), (4, 2)]:
+        want = O.all_pairs_dense(packed, out_bytes, md)
+        d_out = torch.full(((cells + 2) * out_bytes,), 0xA5, dtype=torch.uint8, device="cuda")
+        capi.all_pairs_dev(d_planes, n, packed.shape[1], capi.HD_SEM_DENSE, mask, out_bytes,
+                           md, True, variant, d_out[out_bytes:], stream)
+        torch.cuda.synchronize()
+        got = d_out.cpu().numpy()
+        assert (got[:out_bytes] == 0xA5).all() and (got[-out_bytes:] == 0xA5).all()
+        np.testing.assert_array_equal(got[out_bytes:-out_bytes].view(DT[out_bytes]), want,
+                                      err_msg=f"v={variant} out_bytes={out_bytes} md={md}")
+        for r0, r1 in [(TI, 3 * TI), (2 * TI, n)]:
+            c0, c1 = r0 * (r0 - 1) // 2, r1 * (r1 - 1) // 2
+            d_band = torch.full(((c1 - c0 + 2) * out_bytes,), 0xA5, dtype=torch.uint8,
+                                device="cuda")
+            capi.all_pairs_dev_rows(d_planes, n, packed.shape[1], capi.HD_SEM_DENSE, mask,
+                                    out_bytes, md, True, variant, r0, r1, d_band[out_bytes:],
+                                    stream)
+            torch.cuda.synchronize()
+            got = d_band.cpu().numpy()
+            assert (got[:out_bytes] == 0xA5).all() and (got[-out_bytes:] == 0xA5).all()
+            np.testing.assert_array_equal(got[out_bytes:-out_bytes].view(DT[out_bytes]),
+                                          want[c0:c1], err_msg=f"band {r0}-{r1}")
+
+
 def test_reference_distances_dev_matches_host(rng):
     m, L = 1000, 3000
     seqs = random_strings(rng, m + 1, L, "ACGT-N")
