@@ -199,6 +199,38 @@ def test_dense_brute_force_nibble_rule():
         assert O.distance_dense(pa[0], pb[0]) == want, (a, b)
 
 
+def test_dense_invalid_site_lower_bound(rng):
+    # The early-exit kernels settle a pair before computing it when
+    # max(Z_i, Z_j) reaches the cap, Z the count of invalid (non ACGT-)
+    # characters of a row: under the dense rule an invalid site counts against
+    # every character, '-' and other invalid ones included.  The oracle's
+    # dense distances respect the bound on data with invalid runs at shared
+    # positions, and reach it for a row against itself (its letters and '-'
+    # add nothing).
+    n, L = 60, 400
+    root = rng.choice(list("ACGT-"), size=L)
+    rows = []
+    for k in range(n):
+        s = root.copy()
+        s[rng.integers(0, L, size=k % 4)] = rng.choice(list("ACGT"))
+        a = int(rng.integers(0, L))
+        s[a:a + int(rng.integers(0, 40))] = rng.choice(list("N?X*"))
+        s[rng.integers(0, L, size=int(rng.integers(0, 12)))] = "N"
+        rows.append("".join(s))
+    chars = O.as_chars(rows)
+    packed = O.pack(chars)
+    z = np.array([sum(c not in "ACGT-" for c in r) for r in rows])
+    tri = O.all_pairs_dense(packed, 4, 2**31 - 1)
+    i, j = O.tri_index(n)
+    assert (tri >= np.maximum(z[i], z[j])).all()
+    # invalid sites at the same position count once, so the sum of the two
+    # counts is no bound: the kernels must not use it
+    assert (tri < z[i] + z[j]).any()
+    for r in rows[:10]:
+        p = O.pack(O.as_chars([r, r]))
+        assert O.distance_dense(p[0], p[1]) == sum(c not in "ACGT-" for c in r)
+
+
 def test_all_pairs_dense_layout(rng):
     seqs = random_strings(rng, 40, 33, "ACGT-N")
     p = O.pack(O.as_chars(seqs))
