@@ -620,8 +620,18 @@ __device__ __forceinline__ void allpairs_tile(
   }
 }
 
+// Waves per SIMD a kernel is held to (1: no limit).  The vote-compiled
+// 128 x 64 tiles at 6 (74 VGPRs; the dense 3-op code took 85 and ran 5):
+// 3-7 % faster with the early exit at n = 50,000, L = 29,903, settled or
+// not.  Holding the 128 x 32 ones to 8 (43 VGPRs instead of 66) made their
+// unsettled launches 2-4 % slower.
+template <int R, int C, bool EE>
+constexpr int allpairs_min_waves() {
+  return EE && R == 2 && C == 16 ? 6 : 1;
+}
+
 template <int P, int R, int C, int NW, typename OutT, bool EE>
-__global__ __launch_bounds__(kWave *NW) void k_allpairs(
+__global__ __launch_bounds__(kWave *NW, (allpairs_min_waves<R, C, EE>())) void k_allpairs(
     const uint32_t *__restrict__ planes, uint32_t n, uint32_t W, uint64_t tile_begin,
     uint32_t n_tiles, OutT *__restrict__ out, uint64_t cell_offset, uint32_t max_dist,
     uint32_t check_every, const uint32_t *__restrict__ corr, uint32_t n_pad) {
