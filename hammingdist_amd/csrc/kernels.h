@@ -19,6 +19,7 @@
 
 #include <cstdint>
 #include <type_traits>
+#include <utility>
 
 #include "hammingdist_amd.h"
 
@@ -671,13 +672,67 @@ constexpr int kEncodeWords = 32;
 constexpr int kEncodeStride = kEncodeWords * 16 + 16;
 constexpr int kEncodeLdsBytes = kWave * kEncodeStride;
 
-// Bit p of each of the 8 nibbles of v, gathered into bits 0..7 (nibble k ->
-// bit k): mask the bits, then halve the spacing three times.
-__device__ __forceinline__ uint32_t nibble_bits(uint32_t v, int p) {
-  uint32_t x = (v >> p) & 0x11111111u;
-  x = (x | (x >> 3)) & 0x03030303u;
-  x = (x | (x >> 6)) & 0x000F000Fu;
-  return (x | (x >> 12)) & 0xFFu;
+// Site order inside a plane word.  The 16 bytes of a 32-site word hold site
+// 8 k + j as nibble j of byte word k (k < 4, j < 8).  Exchanging the word
+// index bits with the nibble's bit index (two rounds of masked swaps between
+// word pairs) leaves bit p of that nibble in plane word p at bit 4 j + k:
+// every plane word holds its 32 sites in that fixed order, the same for every
+// row, and nothing outside the encoder depends on it (plane words meet only
+// their counterparts of other rows in bitwise ops and popcounts).
+__device__ __forceinline__ void nibble_planes(const uint32_t (&bytes)[4], uint32_t (&n)[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) n[k] = bytes[k];
+  // a holds the pair's lower word index, b the higher one; bits of a whose
+  // index has `sh` set trade places with bits of b whose index has not.
+  auto swap_bits = [](uint32_t &a, uint32_t &b, int sh, uint32_t m) {
+    const uint32_t t = ((a >> sh) ^ b) & m;
+    b ^= t;
+    a ^= t << sh;
+  };
+  swap_bits(n[0], n[1], 1, 0x55555555u);
+  swap_bits(n[2], n[3], 1, 0x55555555u);
+  swap_bits(n[0], n[2], 2, 0x33333333u);
+  swap_bits(n[1], n[3], 2, 0x33333333u);
+}
+
+// Bits of the first `count` sites of a plane word (count <= 32), in the site
+// order of nibble_planes.
+__device__ __forceinline__ uint32_t first_sites_mask(uint32_t count) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t c = min(8u, count > 8u * k ? count - 8u * k : 0u);
+    m |= (c >= 8 ? 0x11111111u : 0x11111111u & ((1u << (4 * c)) - 1u)) << k;
+  }
+  return m;
+}
+
+// Sites whose nibble is V, from the nibble planes.
+template <int V>
+__device__ __forceinline__ uint32_t nibble_is(const uint32_t (&n)[4]) {
+  return ((V & 1) ? n[0] : ~n[0]) & ((V & 2) ? n[1] : ~n[1]) & ((V & 4) ? n[2] : ~n[2]) &
+         ((V & 8) ? n[3] : ~n[3]);
+}
+
+// Table planes: plane p holds the sites whose nibble v has bit p of tbl[v]
+// set.  Nibble values whose entry is 0 add nothing and are skipped (the
+// table is the same for every lane).
+template <int V>
+__device__ __forceinline__ void table_term(const uint32_t (&n)[4], int P, const uint8_t *s_tbl,
+                                           uint32_t (&w)[6]) {
+  const uint32_t code = __builtin_amdgcn_readfirstlane(s_tbl[V]);
+  if (!code) return;
+  const uint32_t x = nibble_is<V>(n);
+#pragma unroll
+  for (int p = 0; p < 6; ++p)
+    if (p < P && ((code >> p) & 1u)) w[p] |= x;
+}
+
+template <int... V>
+__device__ __forceinline__ void table_planes(const uint32_t (&n)[4], int P, const uint8_t *s_tbl,
+                                             uint32_t (&w)[6],
+                                             std::integer_sequence<int, V...>) {
+  (table_term<V>(n, P, s_tbl, w), ...);
 }
 
 // How the planes are built: through the 16-entry table, as the nibble itself
@@ -686,36 +741,28 @@ __device__ __forceinline__ uint32_t nibble_bits(uint32_t v, int p) {
 enum EncodeMode { kEncTable = 0, kEncIdentity = 1, kEncDense3 = 2 };
 
 // Planes of one 32-site word (16 bytes, sites 2b and 2b + 1 in byte b) into
-// w[0 .. P-1].
+// w[0 .. P-1], sites in the order of nibble_planes.
 __device__ __forceinline__ void encode_word(const uint32_t (&bytes)[4], int P,
                                             const uint8_t *s_tbl, int mode, uint32_t (&w)[6]) {
-  if (mode != kEncTable) {
-    // Plane p of the nibble code is bit p of every nibble, with no table
-    // lookup; kDense3 planes are a few bitwise ops on those.
+  uint32_t n[4];
+  nibble_planes(bytes, n);
+  if (mode == kEncDense3) {
+    // Among 0, A=1, C=2, G=4, T=8 and '-'=15 only '-' has bits 0 and 1.
+    const uint32_t c2 = n[0] ^ n[1];              // A C
+    w[0] = n[0] & n[1];                           // g: '-'
+    w[1] = n[0] | n[2];                           // A G -
+    w[2] = c2;
+    w[3] = c2 | ((n[2] | n[3]) & ~n[0]);          // A C G T
+    return;
+  }
+  if (mode == kEncIdentity) {
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
-      w[p] = nibble_bits(bytes[0], p) | (nibble_bits(bytes[1], p) << 8) |
-             (nibble_bits(bytes[2], p) << 16) | (nibble_bits(bytes[3], p) << 24);
-    if (mode == kEncDense3) {
-      // Among 0, A=1, C=2, G=4, T=8 and '-'=15 only '-' has bits 0 and 1.
-      const uint32_t n0 = w[0], n1 = w[1], n2 = w[2], n3 = w[3];
-      const uint32_t c2 = n0 ^ n1;                   // A C
-      w[0] = n0 & n1;                                // g: '-'
-      w[1] = n0 | n2;                                // A G -
-      w[2] = c2;
-      w[3] = c2 | ((n2 | n3) & ~n0);                 // A C G T
-    }
+    for (int p = 0; p < 4; ++p) w[p] = n[p];
     return;
   }
 #pragma unroll
   for (int p = 0; p < 6; ++p) w[p] = 0;
-#pragma unroll
-  for (int st = 0; st < 32; ++st) {
-    const uint32_t m = s_tbl[(bytes[st >> 3] >> (4 * (st & 7))) & 15u];
-#pragma unroll
-    for (int p = 0; p < 6; ++p)
-      if (p < P) w[p] |= ((m >> p) & 1u) << st;
-  }
+  table_planes(n, P, s_tbl, w, std::make_integer_sequence<int, 16>{});
 }
 
 // The tile (g, k0) of a dataset of n rows of bpr bytes, W words, P stored
@@ -763,7 +810,7 @@ __device__ __forceinline__ void encode_tile(const uint8_t *__restrict__ packed, 
       // kGap*: sites past the packed row leave the '-' plane (their '-'
       // code, the same in every row, stays).
       const uint64_t first = (uint64_t)k * 32, sites = 2 * bpr;
-      if (first + 32 > sites) w[gp] &= first < sites ? (1u << (sites - first)) - 1u : 0u;
+      if (first + 32 > sites) w[gp] &= first < sites ? first_sites_mask(sites - first) : 0u;
     }
     uint32_t *dst = planes + (((uint64_t)g * W + k) * P) * kWave + r;
 #pragma unroll
