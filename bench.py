@@ -765,6 +765,24 @@ def run_api_paths(args, workloads, np):
                        "cells": spec.n * (spec.n - 1) // 2,
                        "fasta_reference_distances_s": t_r}
         log(f"api wide: from_fasta_large {t_w:.2f} s, fasta_reference_distances {t_r:.3f} s")
+        # distance() on two rows of the plain workload (29,903 sites), per call
+        pair = fasta_head(os.path.join(tmp, "plain.fa"), 2) if os.path.exists(
+            os.path.join(tmp, "plain.fa")) else None
+        if pair is not None and len(pair) == 2:
+            a_, b_ = pair
+            d_pair = hd.distance(a_, b_)
+            calls = 500
+            t = time.perf_counter()
+            for _ in range(calls):
+                hd.distance(a_, b_)
+            t_pair = (time.perf_counter() - t) / calls
+
+            def check_pair(O, a_=a_, b_=b_, d_pair=d_pair):
+                assert d_pair == O.distance_char(a_, b_), "api distance(): differs from the oracle"
+
+            defer_check("api distance(): one pair", check_pair)
+            out["distance_call_us"] = {"L": len(a_), "calls": calls, "us_per_call": t_pair * 1e6}
+            log(f"api distance(): {t_pair * 1e6:.0f} us per call (L={len(a_)})")
     finally:
         for f in files:
             if os.path.exists(f):
