@@ -26,6 +26,72 @@ static thread_local std::string g_last_error;
 
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 
+// ---------------------------------------------------------------------------
+// device block pool (DeviceBuffer)
+// ---------------------------------------------------------------------------
+size_t pool_class(size_t bytes) {
+  if (bytes <= (1u << 20)) {
+    size_t c = 256;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  return (bytes + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
+}
+
+namespace {
+struct DevicePool {
+  std::mutex mu;
+  std::map<int, std::multimap<size_t, void *>> free; // per device: class -> blocks
+  std::map<int, size_t> cached;
+};
+// Never destroyed: no HIP call runs during static destruction.
+DevicePool &device_pool() {
+  static DevicePool *p = new DevicePool;
+  return *p;
+}
+int current_device() {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  return d;
+}
+} // namespace
+
+void *pool_take(size_t cls) {
+  const int dev = current_device();
+  {
+    DevicePool &pool = device_pool();
+    std::lock_guard<std::mutex> lock(pool.mu);
+    auto &blocks = pool.free[dev];
+    auto it = blocks.find(cls);
+    if (it != blocks.end()) {
+      void *p = it->second;
+      blocks.erase(it);
+      pool.cached[dev] -= cls;
+      return p;
+    }
+  }
+  void *p = nullptr;
+  HD_HIP(hipMalloc(&p, cls));
+  return p;
+}
+
+void pool_give(void *p, size_t cls) {
+  // The device synchronisation hipFree would perform: work still reading or
+  // writing the block ends before anyone can take it.
+  (void)hipDeviceSynchronize();
+  const int dev = current_device();
+  DevicePool &pool = device_pool();
+  {
+    std::lock_guard<std::mutex> lock(pool.mu);
+    if (pool.cached[dev] + cls <= kPoolCachedMax) {
+      pool.free[dev].emplace(cls, p);
+      pool.cached[dev] += cls;
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
 void ensure_device() {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -1038,6 +1104,135 @@ static void upload(void *dev, const void *host, uint64_t bytes, hipStream_t s) {
   HD_HIP(hipEventSynchronize(st.done[1].get()));
 }
 
+// ---------------------------------------------------------------------------
+// small datasets: one wavefront per pair, straight from the packed rows
+// ---------------------------------------------------------------------------
+// For a few hundred rows the tile kernels are bound by latency, not work:
+// one wavefront walks a row's words in turn (n = 3, L = 29,903: about
+// 0.3 ms of a 0.45-0.59 ms call, with the mask and encode before it).  Here
+// each wavefront takes one pair and its lanes split the row's 32-bit words
+// (8 nibble sites each), every load of a lane issued before the first is
+// used, then a wavefront sum.  The per-site rules are the semantics'
+// definitions on the nibble codes (include/hammingdist_amd.h), evaluated on
+// the 8 nibbles of a word at once (bit 4k of a mask stands for nibble k).
+constexpr uint32_t kNib0 = 0x11111111u;
+
+// Nibbles of v that are not zero.
+__device__ __forceinline__ uint32_t nib_any(uint32_t v) {
+  return (v | (v >> 1) | (v >> 2) | (v >> 3)) & kNib0;
+}
+
+// Nibbles of v that are 15 ('-').
+__device__ __forceinline__ uint32_t nib_gap(uint32_t v) {
+  return v & (v >> 1) & (v >> 2) & (v >> 3) & kNib0;
+}
+
+// The sparse class of each nibble: the letters (1, 2, 4, 8), '-' (15) and,
+// with x_class, X (3) keep their code; every other code becomes the one
+// invalid class 0.
+__device__ __forceinline__ uint32_t sparse_class(uint32_t v, bool x_class) {
+  const uint32_t b0 = v & kNib0, b1 = (v >> 1) & kNib0, b2 = (v >> 2) & kNib0,
+                 b3 = (v >> 3) & kNib0;
+  const uint32_t any = b0 | b1 | b2 | b3;
+  const uint32_t two = (b0 & b1) | (b0 & b2) | (b0 & b3) | (b1 & b2) | (b1 & b3) | (b2 & b3);
+  uint32_t keep = (any & ~two) | (b0 & b1 & b2 & b3);
+  if (x_class) keep |= b0 & b1 & ~b2 & ~b3;
+  return v & (keep * 15u);
+}
+
+template <int SEM>
+__device__ __forceinline__ uint32_t pair_word_count(uint32_t a, uint32_t b) {
+  if constexpr (SEM == HD_SEM_DENSE) return __builtin_popcount(~nib_any(a & b) & kNib0);
+  const uint32_t no_gap = ~(nib_gap(a) | nib_gap(b)) & kNib0;
+  if constexpr (SEM == HD_SEM_CHAR_X)
+    return __builtin_popcount((~nib_any(a & b) & kNib0) | (nib_any(a ^ b) & no_gap));
+  const bool x = SEM == HD_SEM_SPARSE_X;
+  return __builtin_popcount(nib_any(sparse_class(a, x) ^ sparse_class(b, x)) & no_gap);
+}
+
+template <int SEM, typename OutT>
+__global__ __launch_bounds__(256) void k_pairs_small(const uint32_t *__restrict__ packed,
+                                                     uint64_t words, uint64_t pairs,
+                                                     uint32_t max_dist, OutT *__restrict__ out) {
+  const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= pairs) return; // the whole wavefront
+  const uint32_t lane = threadIdx.x & 63;
+  uint64_t i = (uint64_t)((1.0 + sqrt(1.0 + 8.0 * (double)t)) * 0.5);
+  while (i * (i - 1) / 2 > t) --i;
+  while ((i + 1) * i / 2 <= t) ++i;
+  const uint64_t j = t - i * (i - 1) / 2;
+  const uint32_t *ra = packed + i * words, *rb = packed + j * words;
+  constexpr int kIn = 8; // words per lane in flight
+  uint32_t acc = 0;
+  for (uint64_t w0 = lane; w0 < words; w0 += 64 * kIn) {
+    uint32_t a[kIn], b[kIn];
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) {
+      const uint64_t w = min(w0 + 64 * k, words - 1);
+      a[k] = ra[w];
+      b[k] = rb[w];
+    }
+#pragma unroll
+    for (int k = 0; k < kIn; ++k)
+      if (w0 + 64 * k < words) acc += pair_word_count<SEM>(a[k], b[k]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) out[t] = (OutT)min(acc, max_dist);
+}
+
+template <typename OutT>
+static void launch_pairs_small_t(int semantics, const uint8_t *d_packed, uint64_t n,
+                                 uint64_t bpr, uint32_t max_dist, OutT *d_out, hipStream_t s) {
+  const uint64_t pairs = tri_cells(n);
+  const dim3 grid((uint32_t)((pairs + 3) / 4)), block(256);
+  const uint32_t *p = reinterpret_cast<const uint32_t *>(d_packed);
+  const uint64_t words = bpr / 4;
+  switch (semantics) {
+  case HD_SEM_DENSE:
+    hipLaunchKernelGGL((k_pairs_small<HD_SEM_DENSE, OutT>), grid, block, 0, s, p, words, pairs,
+                       max_dist, d_out);
+    break;
+  case HD_SEM_SPARSE:
+    hipLaunchKernelGGL((k_pairs_small<HD_SEM_SPARSE, OutT>), grid, block, 0, s, p, words, pairs,
+                       max_dist, d_out);
+    break;
+  case HD_SEM_SPARSE_X:
+    hipLaunchKernelGGL((k_pairs_small<HD_SEM_SPARSE_X, OutT>), grid, block, 0, s, p, words,
+                       pairs, max_dist, d_out);
+    break;
+  case HD_SEM_CHAR_X:
+    hipLaunchKernelGGL((k_pairs_small<HD_SEM_CHAR_X, OutT>), grid, block, 0, s, p, words, pairs,
+                       max_dist, d_out);
+    break;
+  default:
+    throw Error(HD_ERR_INVALID, "unknown semantics");
+  }
+  HD_HIP(hipGetLastError());
+}
+
+// Datasets the one-pair-per-wavefront kernel takes: at most 512 rows and
+// 1 GiB of row bytes over all pairs (each pair reads both rows, from L2).
+static bool small_pairs(uint64_t n, uint64_t bpr) {
+  return n <= 512 && tri_cells(n) * bpr <= (1ull << 30);
+}
+
+static void launch_pairs_small(int semantics, const uint8_t *d_packed, uint64_t n,
+                               uint64_t bpr, int out_bytes, uint32_t max_dist, void *d_out,
+                               hipStream_t s) {
+  switch (out_bytes) {
+  case 1:
+    return launch_pairs_small_t(semantics, d_packed, n, bpr, max_dist,
+                                static_cast<uint8_t *>(d_out), s);
+  case 2:
+    return launch_pairs_small_t(semantics, d_packed, n, bpr, max_dist,
+                                static_cast<uint16_t *>(d_out), s);
+  default:
+    return launch_pairs_small_t(semantics, d_packed, n, bpr, max_dist,
+                                static_cast<uint32_t *>(d_out), s);
+  }
+}
+
 // Uploads n packed rows and builds their planes (plus the nibble mask).
 struct Encoded {
   Geometry g;
@@ -1201,6 +1396,15 @@ int hd_all_pairs(const uint8_t *packed, uint64_t n, uint64_t bytes_per_seq,
     HD_REQUIRE(packed && host_out, "null buffer");
     ensure_device();
     hipStream_t s = lib_stream();
+    if (small_pairs(n, bytes_per_seq)) {
+      geometry(n, bytes_per_seq, semantics, 0); // argument checks
+      DeviceBuffer d_packed(n * bytes_per_seq), d_cells(tri_cells(n) * out_bytes);
+      upload(d_packed.get(), packed, n * bytes_per_seq, s);
+      launch_pairs_small(semantics, d_packed.get<uint8_t>(), n, bytes_per_seq, out_bytes,
+                         max_dist, d_cells.get(), s);
+      download(host_out, d_cells.get(), tri_cells(n) * out_bytes, s);
+      return;
+    }
     Encoded e;
     upload_encode(e, packed, n, bytes_per_seq, semantics, s);
     const uint64_t cells = tri_cells(n);

@@ -91,10 +91,13 @@ void pack_row(const char *s, std::uint64_t L, const PackTable &t, GeneBlock *out
 PackedRows pack_all(const std::vector<std::string> &data, bool include_x, std::uint64_t bps) {
   PackedRows packed(data.size() * bps);
   const PackTable &t = pack_table(include_x);
-  parallel_for(data.size(), [&](std::uint64_t b, std::uint64_t e) {
-    for (std::uint64_t i = b; i < e; ++i)
-      pack_row(data[i].data(), data[i].size(), t, packed.data() + i * bps, bps);
-  });
+  parallel_for(
+      data.size(),
+      [&](std::uint64_t b, std::uint64_t e) {
+        for (std::uint64_t i = b; i < e; ++i)
+          pack_row(data[i].data(), data[i].size(), t, packed.data() + i * bps, bps);
+      },
+      items_per_worker(2 * bps));
   return packed;
 }
 
@@ -201,14 +204,18 @@ FastaRecords::FastaRecords(const std::string &filename, std::size_t n) {
   }
   spans_ = record_spans(data_, size_, n);
   lengths_.resize(spans_.size());
-  parallel_for(spans_.size(), [&](std::uint64_t b, std::uint64_t e) {
-    for (std::uint64_t k = b; k < e; ++k) {
-      std::uint64_t len = 0;
-      for_each_piece(data_ + spans_[k].begin, spans_[k].end - spans_[k].begin,
-                     [&](const char *, std::uint64_t l) { len += l; });
-      lengths_[k] = len;
-    }
-  });
+  const std::uint64_t per_record = items_per_worker(spans_.empty() ? 1 : size_ / spans_.size());
+  parallel_for(
+      spans_.size(),
+      [&](std::uint64_t b, std::uint64_t e) {
+        for (std::uint64_t k = b; k < e; ++k) {
+          std::uint64_t len = 0;
+          for_each_piece(data_ + spans_[k].begin, spans_[k].end - spans_[k].begin,
+                         [&](const char *, std::uint64_t l) { len += l; });
+          lengths_[k] = len;
+        }
+      },
+      per_record);
 }
 
 FastaRecords::~FastaRecords() {
@@ -303,9 +310,12 @@ bool FastaRecords::same_text(std::size_t a, std::size_t b) const {
 std::vector<std::size_t> FastaRecords::dedup(std::vector<std::size_t> &unique) const {
   const TraceRange trace("dedup");
   std::vector<std::uint64_t> hashes(spans_.size());
-  parallel_for(spans_.size(), [&](std::uint64_t b, std::uint64_t e) {
-    for (std::uint64_t k = b; k < e; ++k) hashes[k] = hash(k);
-  });
+  parallel_for(
+      spans_.size(),
+      [&](std::uint64_t b, std::uint64_t e) {
+        for (std::uint64_t k = b; k < e; ++k) hashes[k] = hash(k);
+      },
+      items_per_worker(spans_.empty() ? 1 : size_ / spans_.size()));
   std::unordered_multimap<std::uint64_t, std::size_t> rows; // hash -> unique row
   rows.reserve(spans_.size());
   std::vector<std::size_t> index;
@@ -331,10 +341,13 @@ std::vector<std::size_t> FastaRecords::dedup(std::vector<std::size_t> &unique) c
 void FastaRecords::pack_rows(const std::vector<std::size_t> &records, const PackTable &t,
                              GeneBlock *out, std::uint64_t bps) const {
   const TraceRange trace("pack rows");
-  parallel_for(records.size(), [&](std::uint64_t b, std::uint64_t e) {
-    std::string scratch;
-    for (std::uint64_t r = b; r < e; ++r) pack(records[r], t, out + r * bps, bps, scratch);
-  });
+  parallel_for(
+      records.size(),
+      [&](std::uint64_t b, std::uint64_t e) {
+        std::string scratch;
+        for (std::uint64_t r = b; r < e; ++r) pack(records[r], t, out + r * bps, bps, scratch);
+      },
+      items_per_worker(2 * bps));
 }
 
 // read_fasta, src/hamming_impl.cc:200-242: skip the first line, read up to n
@@ -347,9 +360,12 @@ FastaData read_fasta(const std::string &filename, bool remove_duplicates, std::s
   for (std::size_t k = 0; k < rows.size(); ++k) rows[k] = k;
   if (remove_duplicates) out.sequence_indices = recs.dedup(rows);
   out.data.resize(rows.size());
-  parallel_for(rows.size(), [&](std::uint64_t b, std::uint64_t e) {
-    for (std::uint64_t k = b; k < e; ++k) out.data[k] = recs.text(rows[k]);
-  });
+  parallel_for(
+      rows.size(),
+      [&](std::uint64_t b, std::uint64_t e) {
+        for (std::uint64_t k = b; k < e; ++k) out.data[k] = recs.text(rows[k]);
+      },
+      items_per_worker(rows.empty() ? 1 : recs.length(rows[0])));
   return out;
 }
 

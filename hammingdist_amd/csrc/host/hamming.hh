@@ -251,7 +251,8 @@ void write_lower_triangular(const std::string &filename, const Cells<T> &cells);
 
 // ---- small parallel helpers -----------------------------------------------
 unsigned worker_count();
-template <typename F> void parallel_for(std::uint64_t n, F &&f);
+template <typename F>
+void parallel_for(std::uint64_t n, F &&f, std::uint64_t min_per_worker = 1);
 
 } // namespace hdhost
 
