@@ -956,9 +956,12 @@ struct RegQuery {
       }
     }
     uint32_t cnt = 0;
+    // Steps whose units all lie past the row end count nothing on any lane
+    // and are skipped (the test is the same for the whole wavefront).
 #pragma unroll
     for (int t = 0; t < T; ++t)
-      cnt += site_count32<SEM>(q[t].x, a[t].x) + site_count32<SEM>(q[t].y, a[t].y);
+      if ((uint32_t)(kWave * t) < units)
+        cnt += site_count32<SEM>(q[t].x, a[t].x) + site_count32<SEM>(q[t].y, a[t].y);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
     return cnt;
