@@ -1211,10 +1211,14 @@ static void launch_pairs_small_t(int semantics, const uint8_t *d_packed, uint64_
   HD_HIP(hipGetLastError());
 }
 
-// Datasets the one-pair-per-wavefront kernel takes: at most 512 rows and
-// 1 GiB of row bytes over all pairs (each pair reads both rows, from L2).
+// Datasets the one-pair-per-wavefront kernel takes: at most 512 rows
+// (HD_SMALL_PAIRS_MAX_ROWS overrides; 0 sends every dataset to the tile
+// kernels) and 1 GiB of row bytes over all pairs (each pair reads both rows,
+// from L2).
 static bool small_pairs(uint64_t n, uint64_t bpr) {
-  return n <= 512 && tri_cells(n) * bpr <= (1ull << 30);
+  uint64_t max_rows = 512;
+  if (const char *e = std::getenv("HD_SMALL_PAIRS_MAX_ROWS")) max_rows = std::strtoull(e, nullptr, 10);
+  return n <= max_rows && tri_cells(n) * bpr <= (1ull << 30);
 }
 
 static void launch_pairs_small(int semantics, const uint8_t *d_packed, uint64_t n,

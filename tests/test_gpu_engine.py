@@ -6,6 +6,8 @@ around the 127/128/255/256/257 boundaries, max_dist {0,1,2,3,89,497,9999999})
 plus sizes that cross this engine's own tile boundaries (64-row groups,
 128/256/512-row tiles, 32-site words).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -18,11 +20,31 @@ pytestmark = pytest.mark.gpu
 DT = {1: np.uint8, 2: np.uint16, 4: np.uint32}
 
 
-def engine_all_pairs(packed, semantics, out_bytes, max_dist):
+def engine_all_pairs(packed, semantics, out_bytes, max_dist, both_paths=True):
+    """hd_all_pairs on host rows.  Datasets of up to 512 rows take the
+    one-pair-per-wavefront kernel; with both_paths they also run with that
+    path switched off (HD_SMALL_PAIRS_MAX_ROWS=0: the tile kernels, as for
+    larger datasets), and the two triangles must be equal."""
     n, bps = packed.shape
-    out = np.zeros(n * (n - 1) // 2, DT[out_bytes])
-    capi.all_pairs(np.ascontiguousarray(packed), n, bps, semantics, out_bytes, max_dist, out)
-    return out
+
+    def run():
+        out = np.zeros(n * (n - 1) // 2, DT[out_bytes])
+        capi.all_pairs(np.ascontiguousarray(packed), n, bps, semantics, out_bytes, max_dist, out)
+        return out
+
+    got = run()
+    if both_paths and n <= 512:
+        old = os.environ.get("HD_SMALL_PAIRS_MAX_ROWS")
+        os.environ["HD_SMALL_PAIRS_MAX_ROWS"] = "0"
+        try:
+            tiles = run()
+        finally:
+            if old is None:
+                del os.environ["HD_SMALL_PAIRS_MAX_ROWS"]
+            else:
+                os.environ["HD_SMALL_PAIRS_MAX_ROWS"] = old
+        np.testing.assert_array_equal(got, tiles, err_msg="one pair per wavefront vs tiles")
+    return got
 
 
 def test_device_present():
@@ -288,11 +310,12 @@ def test_concurrent_host_calls_match_sequential(rng):
         # a few near-duplicates so that the small caps are not all passed at once
         chars[1::7] = chars[0::7][: chars[1::7].shape[0]]
         jobs.append((O.pack(chars), out_bytes, md))
-    alone = [engine_all_pairs(p, capi.HD_SEM_DENSE, ob, md) for p, ob, md in jobs]
+    alone = [engine_all_pairs(p, capi.HD_SEM_DENSE, ob, md, both_paths=False)
+             for p, ob, md in jobs]
     small = jobs[3]
     np.testing.assert_array_equal(alone[3], O.all_pairs_dense(small[0], small[1], small[2]))
     with ThreadPoolExecutor(max_workers=4) as ex:
-        futures = [ex.submit(engine_all_pairs, p, capi.HD_SEM_DENSE, ob, md)
+        futures = [ex.submit(engine_all_pairs, p, capi.HD_SEM_DENSE, ob, md, False)
                    for _ in range(2) for p, ob, md in jobs]
         got = [f.result() for f in futures]
     for k, g in enumerate(got):
