@@ -56,8 +56,9 @@ int current_device() {
 }
 } // namespace
 
-void *pool_take(size_t cls) {
+void *pool_take(size_t cls, int *dev_out) {
   const int dev = current_device();
+  *dev_out = dev;
   {
     DevicePool &pool = device_pool();
     std::lock_guard<std::mutex> lock(pool.mu);
@@ -75,11 +76,13 @@ void *pool_take(size_t cls) {
   return p;
 }
 
-void pool_give(void *p, size_t cls) {
+void pool_give(void *p, size_t cls, int dev) {
   // The device synchronisation hipFree would perform: work still reading or
   // writing the block ends before anyone can take it.
+  const int cur = current_device();
+  if (cur != dev) (void)hipSetDevice(dev);
   (void)hipDeviceSynchronize();
-  const int dev = current_device();
+  if (cur != dev) (void)hipSetDevice(cur);
   DevicePool &pool = device_pool();
   {
     std::lock_guard<std::mutex> lock(pool.mu);

@@ -81,8 +81,10 @@ void ensure_device();
 constexpr size_t kPoolBlockMax = 64ull << 20;
 constexpr size_t kPoolCachedMax = 512ull << 20;
 size_t pool_class(size_t bytes);
-void *pool_take(size_t cls);     // a cached block of the class, or hipMalloc
-void pool_give(void *p, size_t cls);
+// A cached block of the class on the current device (whose id goes to *dev),
+// or a new one; pool_give returns it to that device's blocks.
+void *pool_take(size_t cls, int *dev);
+void pool_give(void *p, size_t cls, int dev);
 
 class DeviceBuffer {
 public:
@@ -95,14 +97,14 @@ public:
     release();
     if (bytes && bytes <= kPoolBlockMax) {
       cls_ = pool_class(bytes);
-      p_ = pool_take(cls_);
+      p_ = pool_take(cls_, &dev_);
     } else if (bytes) {
       HD_HIP(hipMalloc(&p_, bytes));
     }
     n_ = bytes;
   }
   void release() {
-    if (p_ && cls_) pool_give(p_, cls_);
+    if (p_ && cls_) pool_give(p_, cls_, dev_);
     else if (p_) (void)hipFree(p_);
     p_ = nullptr;
     n_ = 0;
@@ -115,6 +117,7 @@ private:
   void *p_ = nullptr;
   size_t n_ = 0;
   size_t cls_ = 0; // pooled block's size class, 0 when not pooled
+  int dev_ = 0;     // pooled block's device
 };
 
 // Pinned host buffer (hipHostMalloc) for asynchronous copies.
