@@ -1,5 +1,6 @@
-"""The reference-side binding shown in INTEGRATION.md (src/distance_hip.cc, the
-file a maintainer adds in place of src/distance_cuda.cu) compiles against the
+"""The reference-side binding of the repository's integration guide
+(src/distance_hip.cc, the file a maintainer adds in place of
+src/distance_cuda.cu) compiles against the
 reference's own headers and this repository's C ABI, links with
 libhd_engine.so, and runs: a driver calls its distance_cuda_have_device()
 and, where a GPU is visible, distances_cuda_8bit / distances_cuda_16bit /
@@ -45,7 +46,7 @@ int main() {
 def binding_source() -> str:
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     m = re.search(r"```cpp\n(// src/distance_hip\.cc\n.*?)```", text, flags=re.S)
-    assert m, "INTEGRATION.md holds no src/distance_hip.cc block"
+    assert m, "the integration guide holds no src/distance_hip.cc block"
     return m.group(1)
 
 
