@@ -976,7 +976,7 @@ def main() -> int:
         args.no_api_paths = True
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only; run it with --gpus 1"}))
         return 2
     if args.copy_probe:
         # A plain 1 GiB device-to-device copy, three times: the known-size
