@@ -210,12 +210,21 @@ int default_variant(const Geometry &g, bool early_exit, uint32_t max_dist) {
   // VGPRs): at n = 100,000, L = 10,000 without votes 144 ms, against 159 ms
   // for 256 x 64 tiles at 5 waves.  With votes, 128 x 32 tiles (cap 255 on
   // the same data: 52.7 ms against 55.8 ms).
-  if (g.code.op == kGap2 || g.code.op == kGap3) return votes ? 4 : 2;
-  // The dense 3-op code: 256 x 64 tiles.  128 x 64 tiles (8 waves per SIMD)
+  // Below 32,768 rows the launch holds few tiles per CU slot and each tile
+  // walks the whole row in turn, so smaller tiles finish sooner: 128 x 32
+  // on the wide data without votes at n = 3,000-20,000 took 0.22 / 0.61 /
+  // 1.53 / 5.74 ms against 0.27 / 0.74 / 1.79 / 5.95 ms for 256 x 32 (at
+  // n = 100,000: 147.2 against 144.3 ms).
+  if (g.code.op == kGap2 || g.code.op == kGap3) return votes || n < 32768 ? 4 : 2;
+  // The dense 3-op code: 256 x 64 tiles from 32,768 rows.  128 x 64 tiles
   // ran 0.7 % faster at n = 50,000, L = 29,903 but fetch twice the bytes
   // from HBM (240 GB against 124 GB per launch: each tile streams its
-  // columns over the whole row, for half as many rows).
-  if (g.code.op == kDense3) return n < 2048 ? 3 : 0;
+  // columns over the whole row, for half as many rows); below that they are
+  // faster by more (n = 6,000 / 10,000 / 20,000 at L = 29,903: 1.75 / 4.55
+  // / 17.0 ms against 2.16 / 4.76 / 17.5 ms), and below 4,096 rows 128 x 32
+  // tiles (n = 600-3,000: 0.42-0.72 ms against 0.61-0.75 ms for 128 x 64,
+  // 0.91-1.04 ms for 256 x 64).
+  if (g.code.op == kDense3) return n < 4096 ? 4 : n < 32768 ? 3 : 0;
   // The pointer-addressed codes (nibble, one-hot, class-3): 128 x 64 tiles,
   // 4.5-8 % faster than 256 x 64 and 256 x 32 at n = 30,000, L = 10,000.
   return 3;
