@@ -123,9 +123,15 @@ def test_default_tile_shape():
     assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 255) == (4, 16)
     # a cap out of reach of the row length turns the votes off
     assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 65535) == (4, 16)
-    assert shape(1000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 255) == (2, 16)
-    # gap codes: 256 x 32 tiles without votes, 128 x 32 with them
+    # fewer rows: smaller tiles, so that the launch holds enough of them
+    assert shape(1000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 255) == (2, 8)
+    assert shape(4095, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 65535) == (2, 8)
+    assert shape(4096, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 65535) == (2, 16)
+    assert shape(32767, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 65535) == (2, 16)
+    assert shape(32768, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 65535) == (4, 16)
+    # gap codes: 256 x 32 tiles without votes from 32,768 rows, else 128 x 32
     assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, False, 65535) == (4, 8)
+    assert shape(20000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, False, 65535) == (2, 8)
     assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, True, 255) == (2, 8)
     assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, True, 3) == (2, 8)
     # the nibble code (dense data holding X) and the one-hot code: 128 x 64
