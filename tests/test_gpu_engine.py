@@ -294,6 +294,40 @@ def test_large_triangle_download_in_chunks(out_bytes, rng):
     np.testing.assert_array_equal(got, want)
 
 
+def test_concurrent_small_calls_match_oracle():
+    # Many short host calls from four threads at once: device blocks come
+    # from and go back to the library's pool between calls, datasets of up to
+    # 512 rows take the one-pair-per-wavefront kernel and larger ones the
+    # tile kernels.  Every result equals the oracle's.
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(606)
+    jobs = []
+    for k in range(48):
+        n = int(rng.choice([2, 3, 17, 64, 130, 511, 512, 513, 700]))
+        L = int(rng.choice([1, 31, 100, 999, 3000]))
+        alphabet = ["ACGT-N", "ACGT-", "ACGTX-N?"][k % 3]
+        chars = O.as_chars(random_strings(rng, n, L, alphabet))
+        sem = [capi.HD_SEM_DENSE, capi.HD_SEM_SPARSE, capi.HD_SEM_SPARSE_X][k % 3]
+        include_x = sem == capi.HD_SEM_SPARSE_X
+        packed = O.pack(chars, include_x)
+        ob, md = [(1, 255), (2, 65535), (4, 7), (1, 2)][k % 4]
+        if sem == capi.HD_SEM_DENSE:
+            want = O.all_pairs_dense(packed, ob, md)
+        else:
+            want = O.all_pairs_sparse(chars, include_x, ob, md)
+        jobs.append((packed, sem, ob, md, want))
+
+    def run(job):
+        packed, sem, ob, md, _ = job
+        return engine_all_pairs(packed, sem, ob, md, both_paths=False)
+
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        got = list(ex.map(run, jobs + jobs))
+    for k, g in enumerate(got):
+        np.testing.assert_array_equal(g, jobs[k % len(jobs)][4], err_msg=f"job {k % len(jobs)}")
+
+
 def test_concurrent_host_calls_match_sequential(rng):
     # Host threads calling the library at once (ctypes releases the GIL):
     # the calls share the library streams and the pinned staging buffers
