@@ -997,6 +997,22 @@ int hd_server_create(const uint8_t *db, uint64_t n_db, uint64_t bytes_per_seq,
     }
     s->dev_mem.alloc(device_bytes);
     s->host_mem.alloc(host_bytes);
+    // Every page of the pinned arena is written once now, from several
+    // threads: its first write costs a page fault, and left to the requests
+    // the first pass over the arena ran at a few GB/s per submitting thread
+    // (4,000 requests from 8 threads: 0.47-0.50 s of submission against 0.17
+    // s once the arena had been written).
+    {
+      const unsigned threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+      uint8_t *h = s->host_mem.get<uint8_t>();
+      std::vector<std::thread> pool;
+      for (unsigned t = 0; t < threads; ++t)
+        pool.emplace_back([=] {
+          const uint64_t b = host_bytes * t / threads, e = host_bytes * (t + 1) / threads;
+          std::memset(h + b, 0, e - b);
+        });
+      for (auto &th : pool) th.join();
+    }
     s->dev_arena.init(s->dev_mem.get<uint8_t>(), device_bytes);
     s->host_arena.init(s->host_mem.get<uint8_t>(), host_bytes);
     // HD_SHARED_MIN_ENTRIES overrides the smallest shared group and drops the
