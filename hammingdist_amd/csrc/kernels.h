@@ -129,11 +129,27 @@ __host__ __device__ constexpr bool has_corr(int P) {
   return P == kDense3 || P == kGap2 || P == kGap3;
 }
 
-// A wave-uniform word copied into a VGPR (one v_mov_b32).
-__device__ __forceinline__ uint32_t to_vgpr(uint32_t x) {
+// A wave-uniform word copied into a VGPR (one v_mov_b32), kept in program
+// order with the other volatile asm of the loop (pin_chains): the copies of
+// a column block are issued right before the block's ops instead of all at
+// the start of the step.
+__device__ __forceinline__ uint32_t to_vgpr_ordered(uint32_t x) {
   uint32_t v;
-  __asm__("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+  __asm__ volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
   return v;
+}
+
+// Marks the accumulators of columns [c0, c0 + G) as written here (no
+// instruction): their popcount-accumulates complete before any later
+// ordered copy, so the scheduler interleaves the R x G independent chains of
+// one column block instead of running long dependent runs, and keeps only
+// that block's column words in VGPRs.
+template <int R, int C, int G>
+__device__ __forceinline__ void pin_chains(uint32_t (&acc)[R][C], int c0) {
+#pragma unroll
+  for (int c = c0; c < c0 + G; ++c)
+#pragma unroll
+    for (int q = 0; q < R; ++q) __asm__ volatile("" : "+v"(acc[q][c]));
 }
 
 // Tile t -> (row block b, column block cb) of a triangle tiled with M column
@@ -454,7 +470,7 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
         // gfx950, with an SGPR operand once.
         uint32_t b[PB];
 #pragma unroll
-        for (int p = 0; p < PB; ++p) b[p] = VCOL ? to_vgpr(bw[c][p]) : bw[c][p];
+        for (int p = 0; p < PB; ++p) b[p] = VCOL ? to_vgpr_ordered(bw[c][p]) : bw[c][p];
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           // v_bitop3 truth tables are over src0 = 0xF0, src1 = 0xCC,
@@ -486,6 +502,14 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
             s = __builtin_amdgcn_bitop3_b32(a[q][P - 1], b[P - 1], s, 0x15);
           }
           acc[q][c] += __builtin_popcount(s);
+        }
+        if constexpr (VCOL) {
+          // Column blocks of 4 rows x 1 column (R >= 4) or 2 rows x 4
+          // columns: at n = 50,000, L = 29,903 the 256 x 64 tiles went from
+          // 101.0 to 97.7 ms, the 128 x 64 ones from 103.4 to 99.1 ms (the
+          // latter 104.5 ms with blocks of 2 x 8, the former 101.4 with 4 x 2).
+          constexpr int G = R >= 4 ? 1 : (4 < C ? 4 : C);
+          if ((c + 1) % G == 0) pin_chains<R, C, G>(acc, c + 1 - G);
         }
       }
       pa += PS * kWave;
