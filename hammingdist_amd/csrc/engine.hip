@@ -217,11 +217,12 @@ int default_variant(const Geometry &g, bool early_exit, uint32_t max_dist) {
   // n = 100,000: 147.2 against 144.3 ms).
   if (g.code.op == kGap2 || g.code.op == kGap3) return votes || n < 32768 ? 4 : 2;
   // The dense 3-op code: 256 x 64 tiles from 32,768 rows.  128 x 64 tiles
-  // ran 0.7 % faster at n = 50,000, L = 29,903 but fetch twice the bytes
-  // from HBM (240 GB against 124 GB per launch: each tile streams its
-  // columns over the whole row, for half as many rows); below that they are
-  // faster by more (n = 6,000 / 10,000 / 20,000 at L = 29,903: 1.75 / 4.55
-  // / 17.0 ms against 2.16 / 4.76 / 17.5 ms), and below 4,096 rows 128 x 32
+  // ran 1.4 % slower at n = 50,000, L = 29,903 (99.1 against 97.7 ms) and
+  // fetch twice the bytes from HBM (240 GB against 124 GB per launch: each
+  // tile streams its columns over the whole row, for half as many rows);
+  // below that they are faster (n = 6,000 / 10,000 / 20,000 at
+  // L = 29,903: 1.75 / 4.55 / 17.0 ms against 2.16 / 4.76 / 17.5 ms, before
+  // the chain interleave took 3-4 % off both), and below 4,096 rows 128 x 32
   // tiles (n = 600-3,000: 0.42-0.72 ms against 0.61-0.75 ms for 128 x 64,
   // 0.91-1.04 ms for 256 x 64).
   if (g.code.op == kDense3) return n < 4096 ? 4 : n < 32768 ? 3 : 0;
