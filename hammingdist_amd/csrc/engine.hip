@@ -185,10 +185,10 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
 // ---------------------------------------------------------------------------
 // Rows per lane, columns per wave, waves per workgroup.
 static const Variant kVariants[] = {
-    {4, 16, 4}, // 256 x 64 tiles: the dense 3-op code at large n
+    {4, 16, 4}, // 256 x 64 tiles: large n without votes
     {8, 8, 4},  // 512 x 32
-    {4, 8, 4},  // 256 x 32: the gap codes (fewer SGPRs per column block)
-    {2, 16, 4}, // 128 x 64: the pointer-addressed codes, n < 2048
+    {4, 8, 4},  // 256 x 32: the gap codes with votes at large n
+    {2, 16, 4}, // 128 x 64: mid-size n, the dense 3-op code with votes
     {2, 8, 4},  // 128 x 32: small datasets, early exit under small caps
 };
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
@@ -199,36 +199,52 @@ const Variant &variant_of(int v) { return kVariants[v]; }
 int default_variant(const Geometry &g, bool early_exit, uint32_t max_dist) {
   const uint64_t n = g.n;
   if (n < 512) return 4;
-  // A cap below kSmallCap is usually passed within the first chunks: a tile
-  // stops at the vote its slowest pair passes, and 128 x 32 tiles stop
-  // sooner than larger ones (n = 50,000, L = 29,903, cap 2: 7.6 ms against
-  // 9.2 ms for 256 x 64).
-  constexpr uint32_t kSmallCap = 64;
   const bool votes = check_interval(early_exit, max_dist, g.W) < g.W;
-  if (votes && max_dist < kSmallCap) return 4;
-  // The kGap codes' kernel runs 256 x 32 tiles at 6 waves per SIMD (77
-  // VGPRs): at n = 100,000, L = 10,000 without votes 144 ms, against 159 ms
-  // for 256 x 64 tiles at 5 waves.  With votes, 128 x 32 tiles (cap 255 on
-  // the same data: 52.7 ms against 55.8 ms).
-  // Below 32,768 rows the launch holds few tiles per CU slot and each tile
-  // walks the whole row in turn, so smaller tiles finish sooner: 128 x 32
-  // on the wide data without votes at n = 3,000-20,000 took 0.22 / 0.61 /
-  // 1.53 / 5.74 ms against 0.27 / 0.74 / 1.79 / 5.95 ms for 256 x 32 (at
-  // n = 100,000: 147.2 against 144.3 ms).
-  if (g.code.op == kGap2 || g.code.op == kGap3) return votes || n < 32768 ? 4 : 2;
-  // The dense 3-op code: 256 x 64 tiles from 32,768 rows.  128 x 64 tiles
-  // ran 1.4 % slower at n = 50,000, L = 29,903 (99.1 against 97.7 ms) and
-  // fetch twice the bytes from HBM (240 GB against 124 GB per launch: each
-  // tile streams its columns over the whole row, for half as many rows);
-  // below that they are faster (n = 6,000 / 10,000 / 20,000 at
-  // L = 29,903: 1.75 / 4.55 / 17.0 ms against 2.16 / 4.76 / 17.5 ms, before
-  // the chain interleave took 3-4 % off both), and below 4,096 rows 128 x 32
-  // tiles (n = 600-3,000: 0.42-0.72 ms against 0.61-0.75 ms for 128 x 64,
-  // 0.91-1.04 ms for 256 x 64).
-  if (g.code.op == kDense3) return n < 4096 ? 4 : n < 32768 ? 3 : 0;
-  // The pointer-addressed codes (nibble, one-hot, class-3): 128 x 64 tiles,
-  // 4.5-8 % faster than 256 x 64 and 256 x 32 at n = 30,000, L = 10,000.
-  return 3;
+  // Below a few thousand rows the launch holds few tiles per CU slot and
+  // each tile walks the whole row in turn, so smaller tiles finish sooner.
+  // Every code copies its column words into VGPRs; the times below are with
+  // those copies (v_bitop3 with VGPR operands only issues twice per CU and
+  // clock, with an SGPR operand once).
+  if (g.code.op == kDense3) {
+    // A cap below kSmallCap is usually passed within the first chunks: a
+    // tile stops at the vote its slowest pair passes, and 128 x 32 tiles
+    // stop sooner than larger ones (n = 50,000, L = 29,903, cap 2: 7.6 ms
+    // against 9.2 ms for 256 x 64, before the invalid-site bound settled
+    // most of those pairs up front).  With a larger cap, 128 x 64 tiles
+    // (cap 255 on the same data: 103.5-105.4 ms against 105.7-105.9 ms for
+    // 256 x 64, whose vote-compiled build runs a wave less per SIMD).
+    constexpr uint32_t kSmallCap = 64;
+    if (votes) return max_dist < kSmallCap || n < 4096 ? 4 : 3;
+    // Without votes, 256 x 64 tiles from 32,768 rows: 128 x 64 tiles ran
+    // 1.4 % slower at n = 50,000, L = 29,903 (99.1 against 97.7 ms) and
+    // fetch twice the bytes from HBM (240 GB against 124 GB per launch: each
+    // tile streams its columns over the whole row, for half as many rows);
+    // below that they are faster (n = 6,000 / 10,000 / 20,000 at
+    // L = 29,903: 1.75 / 4.55 / 17.0 ms against 2.16 / 4.76 / 17.5 ms, before
+    // the chain interleave took 3-4 % off both), and below 4,096 rows 128 x
+    // 32 tiles (n = 600-3,000: 0.42-0.72 ms against 0.61-0.75 ms for 128 x
+    // 64, 0.91-1.04 ms for 256 x 64).
+    return n < 4096 ? 4 : n < 32768 ? 3 : 0;
+  }
+  if (g.code.op == kGap2 || g.code.op == kGap3) {
+    // The wide data (L = 10,000, gap-3 code) without votes: 256 x 64 tiles
+    // from 32,768 rows (n = 100,000: 124.3 ms against 128.8 for 128 x 64
+    // and 133.1 for 256 x 32; n = 50,000: 31.2 / 31.8 / 33.5 ms), 128 x 64
+    // below (n = 6,000 / 10,000 / 20,000: 0.59 / 1.43 / 5.21 ms against
+    // 0.69 / 1.52 / 5.27 for 256 x 64), 128 x 32 below 4,096 (n = 3,000:
+    // 0.21 against 0.25 ms).  With votes, 256 x 32 tiles from 32,768 rows
+    // (n = 100,000, cap 255: 51.6 ms against 53.5 for 128 x 32; cap 3: 16.0
+    // against 16.9), 128 x 32 below (n = 20,000, cap 255: 2.12 against 2.16).
+    if (votes) return n < 32768 ? 4 : 2;
+    return n < 4096 ? 4 : n < 32768 ? 3 : 0;
+  }
+  // The pointer-addressed codes (nibble, one-hot, class-3) on the wide data
+  // at L = 10,000: 256 x 64 tiles from 8,192 rows (n = 30,000: nibble code
+  // 15.1 ms against 15.9 for 128 x 64, one-hot 17.9 against 19.1 for 256 x
+  // 32; n = 10,000: 1.92 / 2.25 ms, tied with or ahead of the others), 128 x
+  // 64 below (n = 6,000: 0.76 against 0.78-0.84 ms), 128 x 32 below 4,096
+  // (n = 3,000: 0.29 against 0.33 ms).
+  return n < 4096 ? 4 : n < 8192 ? 3 : 0;
 }
 
 template <int P, int R, int C, int NW, typename OutT>

@@ -319,11 +319,6 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   constexpr int PA = GAP ? GB : PS;      // row planes used by the main loop
   constexpr int PB = D3 ? 3 : PA;        // column planes used
   constexpr int B0 = D3 ? 1 : 0;         // first column plane used
-  // Column words through VGPRs (one v_mov_b32 per column plane and step):
-  // the dense 3-op code at n = 50,000, L = 29,903 went from 111.0 to 99.5 ms;
-  // the gap-3 code at n = 100,000, L = 10,000 from 144.7 to 222.9 ms (one
-  // wave less per SIMD), so it keeps the SGPR operands.
-  constexpr bool VCOL = D3;
   // Offset codes: accumulators carry OFF + popcounts + the row and column
   // counts added so far.  Rows have fewer than 2^23 sites (kBufWords; the
   // codes are only chosen then), so every partial sum of a pair stays
@@ -465,12 +460,20 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
         for (int c = 0; c < C; ++c) bw[c][p] = pb[(B0 + p) * kWave + c];
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        // VCOL: the column's words are copied into VGPRs first; v_bitop3
-        // with three VGPR operands issues up to twice per CU and clock on
-        // gfx950, with an SGPR operand once.
+        // The column's words are copied into VGPRs first (one v_mov_b32 per
+        // column plane and step): v_bitop3 with three VGPR operands issues
+        // up to twice per CU and clock on gfx950, with an SGPR operand once.
+        // The dense 3-op code at n = 50,000, L = 29,903 went from 111.0 to
+        // 99.5 ms; the gap-3 code at n = 100,000, L = 10,000 from 143.8 to
+        // 124.3 ms and the nibble and one-hot codes at n = 30,000, L =
+        // 10,000 from 17.0 to 15.1 and 20.4 to 17.9 ms, each with its
+        // fastest tile shape before and after (the gap and pointer-addressed
+        // codes gained only once the copies were ordered per column block,
+        // below: issued all at the start of the step they held a VGPR each
+        // for the whole step and cost a wave per SIMD).
         uint32_t b[PB];
 #pragma unroll
-        for (int p = 0; p < PB; ++p) b[p] = VCOL ? to_vgpr_ordered(bw[c][p]) : bw[c][p];
+        for (int p = 0; p < PB; ++p) b[p] = to_vgpr_ordered(bw[c][p]);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           // v_bitop3 truth tables are over src0 = 0xF0, src1 = 0xCC,
@@ -503,14 +506,12 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
           }
           acc[q][c] += __builtin_popcount(s);
         }
-        if constexpr (VCOL) {
-          // Column blocks of 4 rows x 1 column (R >= 4) or 2 rows x 4
-          // columns: at n = 50,000, L = 29,903 the 256 x 64 tiles went from
-          // 101.0 to 97.7 ms, the 128 x 64 ones from 103.4 to 99.1 ms (the
-          // latter 104.5 ms with blocks of 2 x 8, the former 101.4 with 4 x 2).
-          constexpr int G = R >= 4 ? 1 : (4 < C ? 4 : C);
-          if ((c + 1) % G == 0) pin_chains<R, C, G>(acc, c + 1 - G);
-        }
+        // Column blocks of 4 rows x 1 column (R >= 4) or 2 rows x 4 columns:
+        // the dense 3-op code at n = 50,000, L = 29,903 went from 101.0 to
+        // 97.7 ms on 256 x 64 tiles, from 103.4 to 99.1 ms on 128 x 64 (the
+        // latter 104.5 ms with blocks of 2 x 8, the former 101.4 with 4 x 2).
+        constexpr int G = R >= 4 ? 1 : (4 < C ? 4 : C);
+        if ((c + 1) % G == 0) pin_chains<R, C, G>(acc, c + 1 - G);
       }
       pa += PS * kWave;
       pb += PS * kWave;
