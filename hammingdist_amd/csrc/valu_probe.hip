@@ -12,13 +12,16 @@
 //   add        v_add_u32 (a plain 2-input VOP2 op, control)
 //   fma        v_fma_f32 (unpacked; control against the published FP32 rate)
 //   mix31      3 x v_bitop3_b32 (SGPR operand) + 1 x v_bcnt_u32_b32: the dense
-//              3-op and gap-3 tile loops (4 ops per pair and 32-site word)
-//   mix21      2 x v_bitop3_b32 + 1 x v_bcnt_u32_b32: the gap-2 loop
-//   mix41      4 x v_bitop3_b32 + 1 x v_bcnt_u32_b32: the nibble / class-3 loops
+//              3-op and gap-3 pair mix (4 ops per pair and 32-site word) with
+//              the column word as an SGPR operand
+//   mix21      2 x v_bitop3_b32 + 1 x v_bcnt_u32_b32: the gap-2 mix
+//   mix41      4 x v_bitop3_b32 + 1 x v_bcnt_u32_b32: the nibble / class-3 mix
 //   mix31v     3 x v_bitop3_b32 (VGPR operands only) + 1 x v_bcnt_u32_b32
 //   bcast      per column word: 3 x v_mov_b32 from SGPRs, then for 8 rows
 //              3 x v_bitop3_b32 (VGPR operands only) + 1 x v_bcnt_u32_b32
-//              (35 instructions for 32 algorithmic ops; the rate counts the 32)
+//              (35 instructions for 32 algorithmic ops; the rate counts the
+//              32): the form of the tile loops, which copy each column word
+//              into a VGPR before its rows use it
 //
 //   hd_valu_probe [mode|all] [workgroups] [iterations]
 //
