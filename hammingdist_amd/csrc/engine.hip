@@ -118,13 +118,16 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
   case HD_SEM_DENSE:
     pc.planes = 4;
     if ((mask & ~(kLetters | kGapBit | kInvalidBit) & 0xFFFFu) == 0 && words < kBufWords) {
-      // Only 0, A, C, G, T and '-': the 3-op code, planes (g, c1, c2, c3).
+      // Only 0, A, C, G, T and '-': the 3-op code, planes (u, v, w, y)
+      // (the encoder derives them from the nibble bits; the table states
+      // the same code).
       pc.op = kDense3;
-      pc.table[1] = 0xE;  // A: c1 c2 c3
-      pc.table[2] = 0xC;  // C: c2 c3
-      pc.table[4] = 0xA;  // G: c1 c3
-      pc.table[8] = 0x8;  // T: c3
-      pc.table[15] = 0x3; // '-': g c1
+      pc.table[0] = 0x7;  // invalid: u v w
+      pc.table[1] = 0x1;  // A: u
+      pc.table[2] = 0x3;  // C: u v
+      pc.table[4] = 0x8;  // G: y
+      pc.table[8] = 0xA;  // T: v y
+      pc.table[15] = 0x4; // '-': w
       return pc;
     }
     // The nibble itself: f = ((a & b) == 0) for every pair of codes.
@@ -185,10 +188,10 @@ PlaneCode plane_code(int semantics, uint32_t mask, uint32_t words) {
 // ---------------------------------------------------------------------------
 // Rows per lane, columns per wave, waves per workgroup.
 static const Variant kVariants[] = {
-    {4, 16, 4}, // 256 x 64 tiles: large n without votes
+    {4, 16, 4}, // 256 x 64 tiles: large n
     {8, 8, 4},  // 512 x 32
     {4, 8, 4},  // 256 x 32: the gap codes with votes at large n
-    {2, 16, 4}, // 128 x 64: mid-size n, the dense 3-op code with votes
+    {2, 16, 4}, // 128 x 64: mid-size n
     {2, 8, 4},  // 128 x 32: small datasets, early exit under small caps
 };
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
@@ -208,23 +211,26 @@ int default_variant(const Geometry &g, bool early_exit, uint32_t max_dist) {
   if (g.code.op == kDense3) {
     // A cap below kSmallCap is usually passed within the first chunks: a
     // tile stops at the vote its slowest pair passes, and 128 x 32 tiles
-    // stop sooner than larger ones (n = 50,000, L = 29,903, cap 2: 7.6 ms
-    // against 9.2 ms for 256 x 64, before the invalid-site bound settled
-    // most of those pairs up front).  With a larger cap, 128 x 64 tiles
-    // (cap 255 on the same data: 103.5-105.4 ms against 105.7-105.9 ms for
-    // 256 x 64, whose vote-compiled build runs a wave less per SIMD).
+    // stop sooner than larger ones: n = 50,000, L = 29,903 with every 'N'
+    // replaced by 'A' (no row settled by the invalid-site bound), cap 2:
+    // 15.9 ms against 18.5-18.6 for 256 x 32 and 128 x 64 and 26.5 for 256 x
+    // 64; half the row blocks so, 5.1 against 5.8-8.3 ms.  Where the bound
+    // settles every row the launch is a fill of the triangle, which 128 x 32
+    // tiles split finest (0.49-0.51 ms against 0.24-0.29).
     constexpr uint32_t kSmallCap = 64;
-    if (votes) return max_dist < kSmallCap || n < 4096 ? 4 : 3;
-    // Without votes, 256 x 64 tiles from 32,768 rows: 128 x 64 tiles ran
-    // 1.4 % slower at n = 50,000, L = 29,903 (99.1 against 97.7 ms) and
-    // fetch twice the bytes from HBM (240 GB against 124 GB per launch: each
-    // tile streams its columns over the whole row, for half as many rows);
-    // below that they are faster (n = 6,000 / 10,000 / 20,000 at
-    // L = 29,903: 1.75 / 4.55 / 17.0 ms against 2.16 / 4.76 / 17.5 ms, before
-    // the chain interleave took 3-4 % off both), and below 4,096 rows 128 x
-    // 32 tiles (n = 600-3,000: 0.42-0.72 ms against 0.61-0.75 ms for 128 x
-    // 64, 0.91-1.04 ms for 256 x 64).
-    return n < 4096 ? 4 : n < 32768 ? 3 : 0;
+    if (votes && max_dist < kSmallCap) return 4;
+    // Larger caps, and no votes: 256 x 64 tiles for large n, 128 x 64 in
+    // between, 128 x 32 below 8,192 rows (n = 1,000 / 3,000 / 6,000 /
+    // 10,000 / 20,000 / 32,000 / 50,000 at L = 29,903 without votes: 256 x
+    // 64 0.87 / 1.00 / 1.92 / 4.43 / 15.6 / 38.3 / 93.1 ms, 128 x 64 0.59 /
+    // 0.73 / 1.79 / 4.15 / 15.4 / 39.4 / 96.7, 128 x 32 0.45 / 0.61 / 1.70 /
+    // 4.40 / 16.9 / 43.6 / 109.1; with votes at cap 255, 256 x 64 4.36 /
+    // 16.3 / 40.2 / 98.4 ms from 10,000 rows, 128 x 64 4.34 / 16.3 / 41.5 /
+    // 102.1).  128 x 64 tiles also fetch twice the bytes from HBM (240 GB
+    // against 124 GB per launch at n = 50,000: each tile streams its
+    // columns over the whole row, for half as many rows).
+    if (votes) return n < 8192 ? 4 : n < 16384 ? 3 : 0;
+    return n < 8192 ? 4 : n < 24576 ? 3 : 0;
   }
   if (g.code.op == kGap2 || g.code.op == kGap3) {
     // The wide data (L = 10,000, gap-3 code) without votes: 256 x 64 tiles

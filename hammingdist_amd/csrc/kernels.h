@@ -46,15 +46,16 @@ struct Table16 {
 //                one NOR-with-AND and one popcount-accumulate (P + 1 ops).
 //   P = kDense3  the dense nibble semantics, (a & b) == 0, for data whose
 //                nibbles are only 0, A, C, G, T and '-' (the reference's
-//                lookup without X).  Four planes per row, (g, c1, c2, c3):
-//                A (0,1,1,1), C (0,0,1,1), G (0,1,0,1), T (0,0,0,1),
-//                '-' (1,1,0,0), invalid (0,0,0,0).  Per pair and 32 sites
-//                three v_bitop3 (one column plane each) and one
-//                popcount-accumulate, 4 ops:
-//                  s1 = g_i | (c1_i ^ c1_j)
-//                  s2 = c2_i ? (c2_j & ~s1) : ~(c2_j | s1)
-//                  s3 = c3_i ? ~(c3_j & s2) : ~(c3_j | s2)
-//                Per site, f3 + [site of row i invalid] - [site of row j
+//                lookup without X).  Four planes per row, (u, v, w, y):
+//                  u = {A, C, invalid}, v = {C, T, invalid},
+//                  w = {'-', invalid}, y = {G, T};
+//                a row reads u, v, w, a column v, w, y.  Per pair and 32
+//                sites one XOR, two v_bitop3 (one column plane each) and
+//                one popcount-accumulate, 4 ops:
+//                  s1 = v_i ^ v_j
+//                  s2 = ~s1 & (u_i ^ y_j)                    (table 0x14)
+//                  s3 = w_i ? (w_j & ~s2) : (w_j | ~s2)      (table 0x4d)
+//                Per site, s3 + [site of row i invalid] - [site of row j
 //                '-'] is the site's distance term, so popcount(s3) + Z_i - G_j
 //                is the distance, where Z_i counts the invalid sites of row i
 //                and G_j the '-' sites of row j (padding included).  With the
@@ -163,17 +164,18 @@ __device__ __forceinline__ void tile_coords(uint64_t t, uint32_t M, uint32_t &b,
   cb = (uint32_t)(t - (uint64_t)M * r * (r + 1) / 2);
 }
 
-// One 32-site word of the dense 3-op code: row planes (g, c1, c2, c3)
-// against column planes (c1, c2, c3); the popcount of the result plus the
-// row's invalid count minus the column's '-' count is the word's distance
-// (summed over a row).  v_bitop3 truth tables are over src0 = 0xF0, src1 =
-// 0xCC, src2 = 0xAA.
-__device__ __forceinline__ uint32_t dense3_word(uint32_t g, uint32_t c1, uint32_t c2,
-                                                uint32_t c3, uint32_t b1, uint32_t b2,
-                                                uint32_t b3) {
-  uint32_t s = __builtin_amdgcn_bitop3_b32(c1, b1, g, 0xBE); // g | (c1 ^ b1)
-  s = __builtin_amdgcn_bitop3_b32(c2, b2, s, 0x41);
-  return __builtin_amdgcn_bitop3_b32(c3, b3, s, 0x71);
+// One 32-site word of the dense 3-op code: row planes (u, v, w) against
+// column planes (v, w, y); the popcount of the result plus the row's invalid
+// count minus the column's '-' count is the word's distance (summed over a
+// row).  The first op is a plain XOR (a 4-byte VOP2 instruction: the 8-byte
+// v_bitop3 it replaced, g | (c1 ^ b1) on a fourth row plane, made the
+// headline loop 1.7 % slower).  v_bitop3 truth tables are over src0 = 0xF0,
+// src1 = 0xCC, src2 = 0xAA.
+__device__ __forceinline__ uint32_t dense3_word(uint32_t u, uint32_t v, uint32_t w,
+                                                uint32_t bv, uint32_t bw, uint32_t by) {
+  uint32_t s = v ^ bv;
+  s = __builtin_amdgcn_bitop3_b32(u, by, s, 0x14);
+  return __builtin_amdgcn_bitop3_b32(w, bw, s, 0x4d);
 }
 
 // LDS bytes each wave of a staged-store kernel owns.
@@ -316,7 +318,7 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
   constexpr bool OFFSET = D3 || GAP;     // accumulators carry OFF + counts
   constexpr int PS = stored_planes(P);   // planes per row and word
   constexpr int GB = P == kGap2 ? 2 : 3; // kGap*: code bits; plane GB = '-'
-  constexpr int PA = GAP ? GB : PS;      // row planes used by the main loop
+  constexpr int PA = D3 ? 3 : GAP ? GB : PS; // row planes used by the main loop
   constexpr int PB = D3 ? 3 : PA;        // column planes used
   constexpr int B0 = D3 ? 1 : 0;         // first column plane used
   // Offset codes: accumulators carry OFF + popcounts + the row and column
@@ -480,7 +482,7 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
           // src2 = 0xAA.
           uint32_t s;
           if constexpr (D3) {
-            s = dense3_word(a[q][0], a[q][1], a[q][2], a[q][3], b[0], b[1], b[2]);
+            s = dense3_word(a[q][0], a[q][1], a[q][2], b[0], b[1], b[2]);
           } else if constexpr (GAP) {
             // codes differ: XOR, then 0xBE = (x ^ y) | z per further bit
             s = a[q][0] ^ b[0];
@@ -772,12 +774,14 @@ __device__ __forceinline__ void encode_word(const uint32_t (&bytes)[4], int P,
   uint32_t n[4];
   nibble_planes(bytes, n);
   if (mode == kEncDense3) {
-    // Among 0, A=1, C=2, G=4, T=8 and '-'=15 only '-' has bits 0 and 1.
-    const uint32_t c2 = n[0] ^ n[1];              // A C
-    w[0] = n[0] & n[1];                           // g: '-'
-    w[1] = n[0] | n[2];                           // A G -
-    w[2] = c2;
-    w[3] = c2 | ((n[2] | n[3]) & ~n[0]);          // A C G T
+    // Among 0, A=1, C=2, G=4, T=8 and '-'=15 only '-' has bits 0 and 1, and
+    // only the invalid code has no bit.
+    const uint32_t inv = ~(n[0] | n[1] | n[2] | n[3]);
+    const uint32_t gt = (n[2] | n[3]) & ~n[0];    // G T
+    w[0] = (n[0] ^ n[1]) | inv;                   // u: A C invalid
+    w[1] = ((n[1] | n[3]) & ~n[0]) | inv;         // v: C T invalid
+    w[2] = (n[0] & n[1]) | inv;                   // w: '-' invalid
+    w[3] = gt;                                    // y: G T
     return;
   }
   if (mode == kEncIdentity) {
@@ -843,8 +847,8 @@ __device__ __forceinline__ void encode_tile(const uint8_t *__restrict__ packed, 
       if (p < P) dst[p * kWave] = w[p];
     if (corr) {
       if (d3) {
-        ca += __builtin_popcount(~(w[0] | w[1] | w[2] | w[3]));
-        cb += __builtin_popcount(w[0]);
+        ca += __builtin_popcount(w[0] & w[1] & w[2]); // invalid: u, v and w
+        cb += __builtin_popcount(w[2] & ~w[0]);       // '-': w without u
       } else {
         const uint32_t gw = gp == 2 ? w[2] : w[3];
         ca += __builtin_popcount(gw);

@@ -360,12 +360,13 @@ __global__ __launch_bounds__(256) void k_rect_d3(const uint32_t *__restrict__ db
 #pragma unroll
   for (int q = 0; q < R; ++q) row_off[q] = (int)((lane + (uint32_t)(q * group)) * 4);
   for (uint32_t k = 0; k < W; ++k) {
-    uint32_t a[R][PS];
+    // rows: planes u, v, w; columns (the queries): v, w, y
+    uint32_t a[R][3];
     const int step_off = (int)(k * (uint32_t)(PS * kWave * 4));
 #pragma unroll
     for (int q = 0; q < R; ++q)
 #pragma unroll
-      for (int p = 0; p < PS; ++p)
+      for (int p = 0; p < 3; ++p)
         a[q][p] = __builtin_amdgcn_raw_buffer_load_b32(rows_rsrc, row_off[q],
                                                        step_off + p * kWave * 4, 0);
     uint32_t bw[C][3];
@@ -379,8 +380,7 @@ __global__ __launch_bounds__(256) void k_rect_d3(const uint32_t *__restrict__ db
                      b2 = to_vgpr_ordered(bw[c][2]);
 #pragma unroll
       for (int q = 0; q < R; ++q)
-        acc[q][c] += __builtin_popcount(dense3_word(a[q][0], a[q][1], a[q][2], a[q][3], b0,
-                                                    b1, b2));
+        acc[q][c] += __builtin_popcount(dense3_word(a[q][0], a[q][1], a[q][2], b0, b1, b2));
       // blocks of 4 x 1 or 2 x 4 independent chains, as in the all-pairs tiles
       constexpr int G = R >= 4 ? 1 : 4;
       if ((c + 1) % G == 0) pin_chains<R, C, G>(acc, c + 1 - G);

@@ -31,18 +31,30 @@ def sparse_rule(a: str, b: str) -> int:
 
 
 def dense3_planes(nib: int):
-    """Planes (g, c1, c2, c3) from the nibble bits, as the encode kernel
-    derives them."""
+    """Planes (u, v, w, y) from the nibble bits, as the encode kernel derives
+    them: u = {A, C, invalid}, v = {C, T, invalid}, w = {'-', invalid},
+    y = {G, T}."""
     n0, n1, n2, n3 = (nib >> 0) & 1, (nib >> 1) & 1, (nib >> 2) & 1, (nib >> 3) & 1
-    g = n0 & n1
-    c1 = n0 | n2
-    c2 = n0 ^ n1
-    c3 = c2 | ((n2 | n3) & (1 - n0))
-    return g, c1, c2, c3
+    inv = 1 - (n0 | n1 | n2 | n3)
+    u = (n0 ^ n1) | inv
+    v = ((n1 | n3) & (1 - n0)) | inv
+    w = (n0 & n1) | inv
+    y = (n2 | n3) & (1 - n0)
+    return u, v, w, y
 
 
-DENSE3_TABLE = {"A": (0, 1, 1, 1), "C": (0, 0, 1, 1), "G": (0, 1, 0, 1), "T": (0, 0, 0, 1),
-                "-": (1, 1, 0, 0), "inv": (0, 0, 0, 0)}
+DENSE3_TABLE = {"A": (1, 0, 0, 0), "C": (1, 1, 0, 0), "G": (0, 0, 0, 1), "T": (0, 1, 0, 1),
+                "-": (0, 0, 1, 0), "inv": (1, 1, 1, 0)}
+
+
+def dense3_site(a: str, b: str) -> int:
+    """The kernel's chain on one site: row a's planes (u, v, w) against
+    column b's (v, w, y)."""
+    u_i, v_i, w_i, _ = DENSE3_TABLE[a]
+    _, v_j, w_j, y_j = DENSE3_TABLE[b]
+    s = v_i ^ v_j
+    s = bitop3(u_i, y_j, s, 0x14)
+    return bitop3(w_i, w_j, s, 0x4D)
 
 
 def test_dense3_encode_matches_table():
@@ -50,28 +62,60 @@ def test_dense3_encode_matches_table():
         assert dense3_planes(NIBBLE[cls]) == planes, cls
 
 
+def test_dense3_counts_from_planes():
+    # the encoder counts invalid sites as u & v & w and '-' sites as w & ~u
+    for cls, (u, v, w, y) in DENSE3_TABLE.items():
+        assert (u & v & w) == (cls == "inv"), cls
+        assert (w & (1 - u)) == (cls == "-"), cls
+
+
 def test_dense3_chain_with_corrections_is_the_dense_rule():
     classes = list(DENSE3_TABLE)
     for a, b in itertools.product(classes, classes):
-        g_i, c1_i, c2_i, c3_i = DENSE3_TABLE[a]
-        _, c1_j, c2_j, c3_j = DENSE3_TABLE[b]
-        s = bitop3(c1_i, c1_j, g_i, 0xBE)
-        s = bitop3(c2_i, c2_j, s, 0x41)
-        s = bitop3(c3_i, c3_j, s, 0x71)
-        site = s + (a == "inv") - (b == "-")
+        site = dense3_site(a, b) + (a == "inv") - (b == "-")
         assert site == dense_rule(a, b), (a, b)
 
 
 def test_dense3_partial_sums_never_exceed_the_distance_bound():
-    # Per site the term is f3 + [row invalid] - [column '-'] = the dense
+    # Per site the term is s3 + [row invalid] - [column '-'] = the dense
     # rule (0 or 1), so every prefix of sites sums to the exact partial
     # distance; the early-exit vote relies on this.
     classes = list(DENSE3_TABLE)
     for a, b in itertools.product(classes, classes):
-        g_i, c1_i, c2_i, c3_i = DENSE3_TABLE[a]
-        _, c1_j, c2_j, c3_j = DENSE3_TABLE[b]
-        s = bitop3(c3_i, c3_j, bitop3(c2_i, c2_j, bitop3(c1_i, c1_j, g_i, 0xBE), 0x41), 0x71)
-        assert 0 <= s + (a == "inv") - (b == "-") <= 1
+        assert 0 <= dense3_site(a, b) + (a == "inv") - (b == "-") <= 1
+
+
+def test_dense3_chain_catches_a_wrong_table():
+    # a plausible slip (a neighbouring truth table, a wrong plane index, the
+    # chain's operands in another order) breaks the rule on some pair, so the
+    # checks above pin the tables and the operand order
+    classes = list(DENSE3_TABLE)
+
+    def rule_holds(t1, t2, slip=None):
+        for a, b in itertools.product(classes, classes):
+            u_i, v_i, w_i, _ = DENSE3_TABLE[a]
+            _, v_j, w_j, y_j = DENSE3_TABLE[b]
+            if slip == "column v for y":
+                y_j = v_j
+            if slip == "row v for u":
+                u_i = v_i
+            s = v_i ^ v_j
+            if slip == "w before u":
+                s = bitop3(u_i, y_j, bitop3(w_i, w_j, s, t2), t1)
+            else:
+                s = bitop3(w_i, w_j, bitop3(u_i, y_j, s, t1), t2)
+            if slip == "w operands swapped":
+                s = bitop3(w_j, w_i, bitop3(u_i, y_j, v_i ^ v_j, t1), t2)
+            if s + (a == "inv") - (b == "-") != dense_rule(a, b):
+                return False
+        return True
+
+    assert rule_holds(0x14, 0x4D)
+    for slip in ("column v for y", "row v for u", "w before u", "w operands swapped"):
+        assert not rule_holds(0x14, 0x4D, slip), slip
+    for bit in range(8):
+        assert not rule_holds(0x14 ^ (1 << bit), 0x4D)
+        assert not rule_holds(0x14, 0x4D ^ (1 << bit))
 
 
 def class_code_table(present, bits):
