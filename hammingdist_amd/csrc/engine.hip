@@ -219,28 +219,28 @@ int default_variant(const Geometry &g, bool early_exit, uint32_t max_dist) {
     // tiles split finest (0.49-0.51 ms against 0.24-0.29).
     constexpr uint32_t kSmallCap = 64;
     if (votes && max_dist < kSmallCap) return 4;
-    // Larger caps, and no votes: 256 x 64 tiles for large n, 128 x 64 in
-    // between, 128 x 32 below 8,192 rows (n = 1,000 / 3,000 / 6,000 /
-    // 10,000 / 20,000 / 32,000 / 50,000 at L = 29,903 without votes: 256 x
-    // 64 0.87 / 1.00 / 1.92 / 4.43 / 15.6 / 38.3 / 93.1 ms, 128 x 64 0.59 /
-    // 0.73 / 1.79 / 4.15 / 15.4 / 39.4 / 96.7, 128 x 32 0.45 / 0.61 / 1.70 /
-    // 4.40 / 16.9 / 43.6 / 109.1; with votes at cap 255, 256 x 64 4.36 /
-    // 16.3 / 40.2 / 98.4 ms from 10,000 rows, 128 x 64 4.34 / 16.3 / 41.5 /
-    // 102.1).  128 x 64 tiles also fetch twice the bytes from HBM (240 GB
-    // against 124 GB per launch at n = 50,000: each tile streams its
-    // columns over the whole row, for half as many rows).
-    if (votes) return n < 8192 ? 4 : n < 16384 ? 3 : 0;
-    return n < 8192 ? 4 : n < 24576 ? 3 : 0;
+    // Larger caps, and no votes: 128 x 64 tiles from 8,192 rows (256 x 64
+    // with votes from 24,576), 128 x 32 below (L = 29,903 without votes:
+    // n = 6,000 / 20,000 / 50,000 128 x 64 1.65 / 14.7 / 90.4 ms, 128 x 32
+    // 1.61 / 16.1 / 105.3, 256 x 64 1.92 / 15.6 / 92.3; with votes at cap
+    // 255, n = 20,000 / 50,000 128 x 64 15.9 / 99.0 ms, 256 x 64 16.3 /
+    // 97.6).  128 x 64 tiles fetch twice the bytes from HBM that 256 x 64
+    // tiles do (240 GB against 124 GB per launch at n = 50,000: each tile
+    // streams its columns over the whole row, for half as many rows), 2.7
+    // TB/s at that time, a third of the HBM rate.
+    if (votes) return n < 8192 ? 4 : n < 24576 ? 3 : 0;
+    return n < 8192 ? 4 : 3;
   }
   if (g.code.op == kGap2 || g.code.op == kGap3) {
     // The wide data (L = 10,000, gap-3 code) without votes: 256 x 64 tiles
-    // from 32,768 rows (n = 100,000: 124.3 ms against 128.8 for 128 x 64
-    // and 133.1 for 256 x 32; n = 50,000: 31.2 / 31.8 / 33.5 ms), 128 x 64
-    // below (n = 6,000 / 10,000 / 20,000: 0.59 / 1.43 / 5.21 ms against
-    // 0.69 / 1.52 / 5.27 for 256 x 64), 128 x 32 below 4,096 (n = 3,000:
-    // 0.21 against 0.25 ms).  With votes, 256 x 32 tiles from 32,768 rows
-    // (n = 100,000, cap 255: 51.6 ms against 53.5 for 128 x 32; cap 3: 16.0
-    // against 16.9), 128 x 32 below (n = 20,000, cap 255: 2.12 against 2.16).
+    // from 32,768 rows (n = 100,000: 123.6-124.3 ms against 125.0 for 128 x
+    // 64 and 131.7-133.1 for 256 x 32; n = 50,000: 31.2 / 31.8 / 33.5 ms),
+    // 128 x 64 below (n = 6,000 / 10,000 / 20,000: 0.59 / 1.43 / 5.20 ms
+    // against 0.69 / 1.52 / 5.27 for 256 x 64), 128 x 32 below 4,096 (n =
+    // 3,000: 0.21 against 0.25 ms).  With votes, 256 x 32 tiles from 32,768
+    // rows (n = 100,000, cap 255: 51.4-51.7 ms against 51.2-53.5 for 128 x
+    // 32; cap 3: 16.0 against 16.9), 128 x 32 below (n = 20,000, cap 255:
+    // 2.07-2.12 against 2.16).
     if (votes) return n < 32768 ? 4 : 2;
     return n < 4096 ? 4 : n < 32768 ? 3 : 0;
   }

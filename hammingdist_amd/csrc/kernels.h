@@ -473,9 +473,17 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
         // codes gained only once the copies were ordered per column block,
         // below: issued all at the start of the step they held a VGPR each
         // for the whole step and cost a wave per SIMD).
+        // With 2 rows per lane the first XOR of the dense and gap codes takes
+        // its column word as an SGPR operand instead (a VOP2 instruction can):
+        // on 128 x 64 tiles at n = 50,000, L = 29,903 92.6 -> 90.4 ms, on
+        // 128 x 32 tiles at n = 6,000 1.68 -> 1.61 ms; with 4 rows per lane
+        // it was slower (92.5 -> 101.9 ms), and 2-row tiles without any copy
+        // slower still (117.1 ms).
+        constexpr bool SXOR = R == 2 && (D3 || GAP);
         uint32_t b[PB];
 #pragma unroll
-        for (int p = 0; p < PB; ++p) b[p] = to_vgpr_ordered(bw[c][p]);
+        for (int p = 0; p < PB; ++p)
+          b[p] = SXOR && p == 0 ? bw[c][p] : to_vgpr_ordered(bw[c][p]);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           // v_bitop3 truth tables are over src0 = 0xF0, src1 = 0xCC,

@@ -115,22 +115,24 @@ def test_default_tile_shape():
 
     # small datasets: 128 x 32 tiles
     assert shape(300, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 255) == (2, 8)
-    # dense 3-op code comparing every site: 256 x 64 tiles
-    assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 2) == (4, 16)
+    # dense 3-op code comparing every site: 128 x 64 tiles
+    assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 2) == (2, 16)
     # early exit under a small cap: 128 x 32 tiles stop soonest
     assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 2) == (2, 8)
-    # early exit with a cap of 255: 256 x 64 tiles, 128 x 64 from 8,192 rows
+    # early exit with a cap of 255: 256 x 64 tiles from 24,576 rows, 128 x 64
+    # from 8,192 rows
     assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 255) == (4, 16)
     assert shape(10000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 255) == (2, 16)
     assert shape(6000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 255) == (2, 8)
     # a cap out of reach of the row length turns the votes off
-    assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 65535) == (4, 16)
+    assert shape(50000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 65535) == (2, 16)
     # fewer rows: smaller tiles, so that the launch holds enough of them
     assert shape(1000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 255) == (2, 8)
     assert shape(8191, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 65535) == (2, 8)
     assert shape(8192, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 65535) == (2, 16)
-    assert shape(24575, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 65535) == (2, 16)
-    assert shape(24576, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 65535) == (4, 16)
+    assert shape(100000, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, False, 65535) == (2, 16)
+    assert shape(24575, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 255) == (2, 16)
+    assert shape(24576, 14952, capi.HD_SEM_DENSE, acgt_gap_inv, True, 255) == (4, 16)
     # gap codes without votes: 256 x 64 tiles from 32,768 rows, 128 x 64 from
     # 4,096, else 128 x 32; with votes 256 x 32 from 32,768 rows, else 128 x 32
     assert shape(100000, 5000, capi.HD_SEM_SPARSE_X, 0xFFFF, False, 65535) == (4, 16)
