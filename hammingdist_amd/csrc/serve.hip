@@ -376,8 +376,10 @@ __global__ __launch_bounds__(256) void k_rect_d3(const uint32_t *__restrict__ db
       for (int c = 0; c < C; ++c) bw[c][p] = pb[(1 + p) * kWave + c];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const uint32_t b0 = to_vgpr_ordered(bw[c][0]), b1 = to_vgpr_ordered(bw[c][1]),
-                     b2 = to_vgpr_ordered(bw[c][2]);
+      // with 2 rows per lane the XOR reads its column word from the SGPR,
+      // as in the all-pairs tiles
+      const uint32_t b0 = R == 2 ? bw[c][0] : to_vgpr_ordered(bw[c][0]),
+                     b1 = to_vgpr_ordered(bw[c][1]), b2 = to_vgpr_ordered(bw[c][2]);
 #pragma unroll
       for (int q = 0; q < R; ++q)
         acc[q][c] += __builtin_popcount(dense3_word(a[q][0], a[q][1], a[q][2], b0, b1, b2));
