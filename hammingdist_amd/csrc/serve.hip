@@ -302,20 +302,20 @@ static const void *shared_scan_kernel(int semantics, uint64_t bpr) {
 // and one launch computes the distance of every (database row, query) pair
 // of the group: workgroup b holds database rows [64 R b, 64 R (b + 1)) in
 // its lanes' VGPRs (R rows per lane), wavefront w queries [16 w, 16 w + 16)
-// in SGPRs copied to VGPRs, 3 v_bitop3 + 1 v_bcnt per pair and 32 sites
-// (kernels.h, tile_wave).
+// in SGPRs (two of each query's three words copied to VGPRs), 1 v_xor + 2
+// v_bitop3 + 1 v_bcnt per pair and 32 sites (kernels.h, dense3_word).
 // The distances land in a query-major matrix; a gather then reads each
 // query's row positions out of it.  At 32 queries against 1,000,000 rows of
 // 29,903 sites this is about 2.6 ms, where the queries' own scans of 215,000
 // rows each take 17 ms.
 constexpr uint32_t kRectMaxQ = 64;
 constexpr uint32_t kRectCols = 16; // queries per wavefront
-// Groups of at most this many queries take 2 database rows per lane (128-row
-// workgroups, 48 VGPRs, up to 8 wavefronts per SIMD): with few queries the
-// tile loop waits on its row loads, and twice the wavefronts keep more of
-// them in flight; larger groups take 4 rows per lane (256-row workgroups,
-// 92 VGPRs), a copied query word serving 4 rows.
-constexpr uint32_t kRectSmallQ = 32;
+// Every group takes 2 database rows per lane (128-row workgroups, up to 8
+// wavefronts per SIMD), the XOR reading its query word from the SGPR: over
+// 200,000 rows a group of 64 queries took 1.21 ms against 1.66 ms with 4
+// rows per lane (256-row workgroups, each copied query word serving 4 rows),
+// groups of up to 32 queries 0.78-0.79 ms; with few queries the loop waits
+// on its row loads, and the smaller rows' wavefronts keep more in flight.
 // Nibble codes the dense 3-op code encodes: invalid (0), A C G T, '-'.
 constexpr uint32_t kDense3Codes = (1u << 0) | (1u << 1) | (1u << 2) | (1u << 4) | (1u << 8) |
                                   (1u << 15);
@@ -778,11 +778,10 @@ struct hd_server {
         // not depend on which of its codes are present).
         const Geometry gq = geometry(n_g, bpr, HD_SEM_DENSE, db_mask);
         launch_encode(gq, S.q_packed.get<uint8_t>(), S.q_planes.get<uint32_t>(), s);
-        const bool small = n_g <= kRectSmallQ;
-        const uint64_t rows_per_block = small ? 128 : 256;
+        constexpr uint64_t rows_per_block = 128;
         const dim3 grid((uint32_t)((n_db + rows_per_block - 1) / rows_per_block),
                         (n_g + 4 * kRectCols - 1) / (4 * kRectCols));
-        hipLaunchKernelGGL(small ? k_rect_d3<2> : k_rect_d3<4>, grid, dim3(256), 0, s,
+        hipLaunchKernelGGL(k_rect_d3<2>, grid, dim3(256), 0, s,
                            db_planes.get<uint32_t>(), (uint32_t)n_db, (uint32_t)g_db.n_pad,
                            g_db.W, S.q_planes.get<uint32_t>(), n_g, (uint32_t)gq.n_pad,
                            S.dist.get<uint32_t>(), n_db);
