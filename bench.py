@@ -57,7 +57,7 @@ def valu_peak(dual: int, single: int) -> float:
 # Algorithmic VALU ops per pair and 32-site word, as (dual-rate, single-rate):
 # every code is a chain of bitwise ops (v_bitop3 / v_xor / v_and, dual-rate
 # with VGPR operands) and one popcount-accumulate (v_bcnt, single-rate).
-#   the dense 3-op code: three v_bitop3 + v_bcnt
+#   the dense 3-op code: one XOR, two v_bitop3 + v_bcnt
 #   the nibble code with P planes: P bitwise ops + v_bcnt
 #   the sparse gap codes with P stored planes (P - 1 code bits): one XOR,
 #   P - 2 v_bitop3 + v_bcnt (the 2 |g_i & g_j| pass over words holding a '-'
