@@ -37,8 +37,11 @@ struct Table16 {
 // A workgroup of NW waves owns a TI x TJ tile of the lower triangle
 // (TI = 64 R rows, TJ = NW C columns).  Lane l of every wave holds rows
 // i0 + l + 64 q (q < R) in VGPRs; wave w holds columns j0 + w C + c (c < C)
-// in SGPRs (wave-uniform scalar loads), so each VALU op pairs a per-lane row
-// word with a broadcast column word and no LDS traffic is needed.
+// in SGPRs (wave-uniform scalar loads), copied into VGPRs one column at a
+// time right before the column's R rows use them (the first XOR of the dense
+// and gap codes with R = 2 reads its word from the SGPR), so each VALU op
+// pairs a per-lane row word with a broadcast column word and no LDS traffic
+// is needed.
 //
 // Plane codes (template parameter P):
 //   P = 4, 5, 6  P one-hot-style planes; a site counts when no plane is set
