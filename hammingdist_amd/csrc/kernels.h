@@ -38,8 +38,8 @@ struct Table16 {
 // (TI = 64 R rows, TJ = NW C columns).  Lane l of every wave holds rows
 // i0 + l + 64 q (q < R) in VGPRs; wave w holds columns j0 + w C + c (c < C)
 // in SGPRs (wave-uniform scalar loads), copied into VGPRs one column at a
-// time right before the column's R rows use them (the first XOR of the dense
-// and gap codes with R = 2 reads its word from the SGPR), so each VALU op
+// time right before the column's R rows use them (the first op of every code
+// reads its word from the SGPR when R <= 2), so each VALU op
 // pairs a per-lane row word with a broadcast column word and no LDS traffic
 // is needed.
 //
@@ -476,13 +476,15 @@ __device__ __forceinline__ void tile_wave(const uint32_t *__restrict__ planes, u
         // codes gained only once the copies were ordered per column block,
         // below: issued all at the start of the step they held a VGPR each
         // for the whole step and cost a wave per SIMD).
-        // With 2 rows per lane the first XOR of the dense and gap codes takes
-        // its column word as an SGPR operand instead (a VOP2 instruction can):
-        // on 128 x 64 tiles at n = 50,000, L = 29,903 92.6 -> 90.4 ms, on
-        // 128 x 32 tiles at n = 6,000 1.68 -> 1.61 ms; with 4 rows per lane
-        // it was slower (92.5 -> 101.9 ms), and 2-row tiles without any copy
-        // slower still (117.1 ms).
-        constexpr bool SXOR = R == 2 && (D3 || GAP);
+        // With at most 2 rows per lane every code's first op (an XOR or AND,
+        // a VOP2 instruction) takes its column word as an SGPR operand
+        // instead: the dense code on 128 x 64 tiles at n = 50,000, L = 29,903
+        // 92.6 -> 90.4 ms, on 128 x 32 tiles at n = 6,000 1.68 -> 1.61 ms;
+        // the nibble and one-hot codes on 128 x 32 tiles at n = 1,000-3,000,
+        // L = 10,000 2-3 % faster; with 4 rows per lane it was slower (92.5
+        // -> 101.9 ms), and 2-row tiles without any copy slower still (117.1
+        // ms).
+        constexpr bool SXOR = R <= 2;
         uint32_t b[PB];
 #pragma unroll
         for (int p = 0; p < PB; ++p)
