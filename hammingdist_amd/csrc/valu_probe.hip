@@ -22,6 +22,12 @@
 //              (35 instructions for 32 algorithmic ops; the rate counts the
 //              32): the form of the tile loops, which copy each column word
 //              into a VGPR before its rows use it
+//   bcast2     per block of 4 column words: 2 x v_mov_b32 from SGPRs per
+//              word, then for 2 rows of each word v_xor_b32 with the third
+//              word as an SGPR operand, 2 x v_bitop3_b32 (VGPR operands) and
+//              v_bcnt_u32_b32, the block's 8 chains independent (40
+//              instructions for 32 algorithmic ops): the form of the dense
+//              and gap tile loops with 2 rows per lane
 //
 //   hd_valu_probe [mode|all] [workgroups] [iterations]
 //
@@ -44,7 +50,7 @@
 constexpr int kChains = 8;
 
 enum Mode {
-  BITOP3_S, BITOP3_V, BCNT, ADD, FMA, MIX31, MIX21, MIX41, MIX31V, BCAST,
+  BITOP3_S, BITOP3_V, BCNT, ADD, FMA, MIX31, MIX21, MIX41, MIX31V, BCAST, BCAST2,
   N_MODES
 };
 
@@ -56,7 +62,7 @@ struct ModeInfo {
 constexpr ModeInfo kModes[N_MODES] = {
     {"bitop3_s", 4}, {"bitop3_v", 4}, {"bcnt", 4}, {"add", 4},
     {"fma", 4},      {"mix31", 4},    {"mix21", 3}, {"mix41", 5},
-    {"mix31v", 4},   {"bcast", 4},
+    {"mix31v", 4},   {"bcast", 4},    {"bcast2", 4},
 };
 
 __device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
@@ -152,6 +158,23 @@ __global__ __launch_bounds__(256) void k_valu(uint32_t seed, uint32_t iters,
             acc[r] = bcnt_acc(a[r], acc[r]);
           }
         }
+      } else if constexpr (M == BCAST2) {
+        if (k == 0) {
+          // a block of 4 column words x 2 rows, its 8 chains independent
+          uint32_t m2[kChains / 2], m3[kChains / 2];
+#pragma unroll
+          for (int c = 0; c < kChains / 2; ++c) {
+            asm volatile("v_mov_b32 %0, %1" : "=v"(m2[c]) : "s"(b1));
+            asm volatile("v_mov_b32 %0, %1" : "=v"(m3[c]) : "s"(b2));
+          }
+#pragma unroll
+          for (int q = 0; q < kChains; ++q) {
+            uint32_t t = a[q] ^ b0; // VOP2 with the SGPR operand
+            t = __builtin_amdgcn_bitop3_b32(a[q], m3[q / 2], t, 0x14);
+            a[q] = __builtin_amdgcn_bitop3_b32(t, m2[q / 2], a[q], 0x4d);
+            acc[q] = bcnt_acc(a[q], acc[q]);
+          }
+        }
       }
     }
   }
@@ -178,6 +201,7 @@ static KernelFn kernel_of(int m) {
   case MIX21: return k_valu<MIX21>;
   case MIX41: return k_valu<MIX41>;
   case MIX31V: return k_valu<MIX31V>;
+  case BCAST2: return k_valu<BCAST2>;
   default: return k_valu<BCAST>;
   }
 }
